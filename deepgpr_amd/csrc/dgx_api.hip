@@ -1,0 +1,967 @@
+// Host drivers of the forward and adjoint time loops and the exported C ABI.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "deepgpr_amd.h"
+#include "dgx_internal.h"
+
+using namespace dgx;
+
+namespace {
+
+thread_local char g_error[512];
+int g_abi6_order = 2;  // set_fdtd_order(), read by the ABI-6 adapters only
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+}
+
+// Device allocation released at scope exit (after the stream has drained).
+struct DevBuf {
+    void* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    bool alloc(size_t bytes)
+    {
+        if (bytes == 0) bytes = 16;
+        return hipMalloc(&p, bytes) == hipSuccess;
+    }
+};
+
+#define DGX_TRY(call, what)                                                         \
+    do {                                                                            \
+        hipError_t err_ = (call);                                                   \
+        if (err_ != hipSuccess) {                                                   \
+            set_error("%s: %s", (what), hipGetErrorString(err_));                   \
+            return -1;                                                              \
+        }                                                                           \
+    } while (0)
+
+constexpr int kFlagRhsFromE = 1 << 4;
+constexpr int kFlagPhysical = 1 << 5;
+constexpr int kFlagNoHistory = 1 << 26;
+
+struct Plan {
+    Grid g;
+    int order;
+    int kind;
+    bool history;
+    bool e_only;
+    bool store_r;
+    bool physical;
+    int tile[3];
+    Box box;
+    int comps;
+    int nt_saved;
+    long long snap;     // elements of one frame of one component (all shots)
+    long long cstride;  // nt_saved * snap
+    bool has_cpml;
+    Cpml cpe, cph;
+};
+
+void box_axis(bool physical, int cells, int lo, int hi, int tile, int* o, int* e)
+{
+    if (!physical) {
+        *o = 0;
+        *e = cells;
+        return;
+    }
+    int begin = (lo / tile) * tile;
+    int end = ((cells - hi + tile - 1) / tile) * tile;
+    if (end > cells) end = cells;
+    *o = begin;
+    *e = end - begin;
+}
+
+void host_psi_dims(const Grid& g, bool electric, int face, int w, int dims[3])
+{
+    static const int target[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+    static const int source[3][2] = {{2, 1}, {2, 0}, {1, 0}};
+    int axis = face / 2, p = g.pml[face];
+    for (int a = 0; a < 3; ++a) dims[a] = g.N[a];
+    if (electric) {
+        dims[target[axis][w]] -= 1;
+        dims[axis] = p + 1;
+    } else {
+        dims[source[axis][w]] -= 1;
+        dims[axis] = p;
+    }
+}
+
+long long psi_elems(const Grid& g, bool electric, int face, int w)
+{
+    if (g.pml[face] <= 0) return 0;
+    int d[3];
+    host_psi_dims(g, electric, face, w, d);
+    return (long long)g.nshot * d[0] * d[1] * d[2];
+}
+
+int make_plan(const dgx_solve* P, bool backward, Plan& pl)
+{
+    if (!P) {
+        set_error("null solve descriptor");
+        return -1;
+    }
+    if (P->abi_version != DGX_ABI_VERSION) {
+        set_error("descriptor abi_version %d != %d", P->abi_version, DGX_ABI_VERSION);
+        return -1;
+    }
+    Grid& g = pl.g;
+    g.N[0] = P->nx_fields;
+    g.N[1] = P->ny_fields;
+    g.N[2] = P->nz_fields;
+    if (g.N[0] < 2 || g.N[1] < 2 || g.N[2] < 2) {
+        set_error("field sizes must be >= 2 (got %d, %d, %d)", g.N[0], g.N[1], g.N[2]);
+        return -1;
+    }
+    if ((long long)g.N[1] * g.N[2] > (1LL << 30)) {
+        set_error("y-z plane of %lld cells is too large", (long long)g.N[1] * g.N[2]);
+        return -1;
+    }
+    if (P->nshot < 1 || P->nshot > 65535) {
+        set_error("nshot must be in [1, 65535] (got %d)", P->nshot);
+        return -1;
+    }
+    if (P->nt < 0 || P->sampling_interval < 1) {
+        set_error("nt must be >= 0 and sampling_interval >= 1");
+        return -1;
+    }
+    g.st0 = (long long)g.N[1] * g.N[2];
+    g.plane = g.N[1] * g.N[2];
+    g.cells = (long long)g.N[0] * g.st0;
+    g.nshot = P->nshot;
+    g.d[0] = P->dx;
+    g.d[1] = P->dy;
+    g.d[2] = P->dz;
+    pl.has_cpml = false;
+    for (int f = 0; f < 6; ++f) {
+        g.pml[f] = P->pml[f];
+        if (g.pml[f] < 0) {
+            set_error("negative PML thickness on face %d", f);
+            return -1;
+        }
+        if (g.pml[f]) pl.has_cpml = true;
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (g.pml[2 * a] + g.pml[2 * a + 1] > g.N[a] - 2) {
+            set_error("PML thickness %d + %d leaves no physical cell on axis %d (field size %d)", g.pml[2 * a],
+                      g.pml[2 * a + 1], a, g.N[a]);
+            return -1;
+        }
+    }
+    pl.order = (P->fdtd_order == 4 || P->fdtd_order == 8) ? P->fdtd_order : 2;
+    int st = P->storage_type;
+    pl.kind = st & 0x0f;
+    if (pl.kind > kInt8) {
+        set_error("unknown wavefield storage kind %d", pl.kind);
+        return -1;
+    }
+    bool want_mat = P->eps_r_requires_grad || P->sigma_requires_grad;
+    if (!backward) {
+        pl.history = (st & kFlagNoHistory) == 0;
+        pl.e_only = pl.history && P->save_model_history && (st & kFlagRhsFromE);
+        pl.store_r = pl.history && P->save_model_history && !pl.e_only;
+    } else {
+        pl.history = want_mat;
+        pl.e_only = want_mat && (st & kFlagRhsFromE);
+        pl.store_r = !pl.e_only;
+    }
+    pl.physical = (st & kFlagPhysical) != 0;
+    if (pl.kind == kInt8) {
+        pl.tile[0] = (st >> 8) & 0x3f;
+        pl.tile[1] = (st >> 14) & 0x3f;
+        pl.tile[2] = (st >> 20) & 0x3f;
+        for (int a = 0; a < 3; ++a)
+            if (pl.tile[a] <= 0) {
+                set_error("INT8 storage_type carries an empty block on axis %d", a);
+                return -1;
+            }
+        int vol = pl.tile[0] * pl.tile[1] * pl.tile[2];
+        if (vol > 256 || (vol & (vol - 1))) {
+            set_error("INT8 block volume %d must be a power of two <= 256", vol);
+            return -1;
+        }
+    } else {
+        pl.tile[0] = pl.tile[1] = pl.tile[2] = 1;
+    }
+    Box& b = pl.box;
+    box_axis(pl.physical, g.N[0] - 1, g.pml[0], g.pml[1], pl.tile[0], &b.x0, &b.nx);
+    box_axis(pl.physical, g.N[1] - 1, g.pml[2], g.pml[3], pl.tile[1], &b.y0, &b.ny);
+    box_axis(pl.physical, g.N[2] - 1, g.pml[4], g.pml[5], pl.tile[2], &b.z0, &b.nz);
+    b.cells = (long long)b.nx * b.ny * b.nz;
+    pl.comps = P->fwi_mode == 3 ? 3 : 1;
+    pl.nt_saved = (P->nt + P->sampling_interval - 1) / P->sampling_interval;
+    pl.snap = (long long)g.nshot * b.cells;
+    pl.cstride = (long long)pl.nt_saved * pl.snap;
+
+    const char* who = backward ? "backward" : "forward";
+    bool needs_mat = backward ? want_mat : (pl.history && P->save_model_history);
+    if (needs_mat && pl.has_cpml && pl.physical) {
+        set_error("%s: material gradients with PML require extended wavefield history; physical history omits "
+                  "PML material sensitivities", who);
+        return -1;
+    }
+    if (pl.e_only && P->sampling_interval != 1) {
+        set_error("%s: E-only wavefield history requires sampling interval 1", who);
+        return -1;
+    }
+    for (int f = 0; f < 6; ++f) {
+        pl.cpe.coef[f] = P->pml_coeff[f];
+        pl.cph.coef[f] = P->pml_coeff[6 + f];
+        pl.cpe.psi[f][0] = P->phi[4 * f + 0];
+        pl.cpe.psi[f][1] = P->phi[4 * f + 1];
+        pl.cph.psi[f][0] = P->phi[4 * f + 2];
+        pl.cph.psi[f][1] = P->phi[4 * f + 3];
+        if (g.pml[f] > 0 && (!pl.cpe.coef[f] || !pl.cph.coef[f] || !pl.cpe.psi[f][0] || !pl.cpe.psi[f][1] ||
+                             !pl.cph.psi[f][0] || !pl.cph.psi[f][1])) {
+            set_error("%s: CPML face %d has missing arrays", who, f);
+            return -1;
+        }
+    }
+    for (int c = 0; c < 3; ++c)
+        if (!P->E[c] || !P->H[c]) {
+            set_error("%s: null field pointer", who);
+            return -1;
+        }
+    for (int c = 0; c < 6; ++c)
+        if (!P->update_coeff[c]) {
+            set_error("%s: null update-coefficient scratch", who);
+            return -1;
+        }
+    return 0;
+}
+
+bool loc_ok(const Grid& g, const int* L)
+{
+    return L[0] >= 0 && L[0] < g.N[0] && L[1] >= 0 && L[1] < g.N[1] && L[2] >= 0 && L[2] < g.N[2];
+}
+
+size_t elem_bytes(int kind) { return kind == kF32 ? 4 : (kind == kInt8 ? 1 : 2); }
+
+struct Int8Geom {
+    int nbx, nby, nbz;
+    long long tiles;
+};
+
+Int8Geom int8_geom(const Plan& pl)
+{
+    Int8Geom q;
+    q.nbx = (pl.box.nx + pl.tile[0] - 1) / pl.tile[0];
+    q.nby = (pl.box.ny + pl.tile[1] - 1) / pl.tile[1];
+    q.nbz = (pl.box.nz + pl.tile[2] - 1) / pl.tile[2];
+    q.tiles = (long long)q.nbx * q.nby * q.nbz;
+    return q;
+}
+
+long long aligned4(long long v) { return (v + 3) & ~3LL; }
+
+Int8Write int8_writer(const Plan& pl, void* packed, int frames, int frame)
+{
+    Int8Geom q = int8_geom(pl);
+    Int8Write w;
+    w.q = (signed char*)packed;
+    w.scale = (float*)((unsigned char*)packed + aligned4((long long)pl.comps * frames * pl.snap));
+    w.frames = frames;
+    w.frame = frame;
+    w.bx = pl.tile[0];
+    w.by = pl.tile[1];
+    w.bz = pl.tile[2];
+    w.nbx = q.nbx;
+    w.nby = q.nby;
+    w.nbz = q.nbz;
+    w.tiles = q.tiles;
+    w.mode = pl.comps == 3 ? 3 : 2;
+    w.box = pl.box;
+    return w;
+}
+
+Int8View int8_view(const Plan& pl, const void* packed, int frames, int frame)
+{
+    Int8Geom q = int8_geom(pl);
+    Int8View v;
+    v.q = (const signed char*)packed + (long long)frame * pl.snap;
+    v.q_cstride = (long long)frames * pl.snap;
+    v.scale = (const float*)((const unsigned char*)packed + aligned4((long long)pl.comps * frames * pl.snap)) +
+              (long long)frame * pl.g.nshot * q.tiles;
+    v.s_cstride = (long long)frames * pl.g.nshot * q.tiles;
+    v.bx = pl.tile[0];
+    v.by = pl.tile[1];
+    v.bz = pl.tile[2];
+    v.nby = q.nby;
+    v.nbz = q.nbz;
+    v.tiles = q.tiles;
+    return v;
+}
+
+int run_forward(const dgx_solve* P, hipStream_t stream)
+{
+    Plan pl;
+    if (make_plan(P, false, pl)) return -1;
+    const Grid& g = pl.g;
+    const int nt = P->nt;
+    const int S = P->sampling_interval;
+    float* const* co = P->update_coeff;
+
+    // --- sources: group entries by the block of the electric sweep ---------
+    dim3 grid = cell_grid(g);
+    std::vector<int> hsloc((size_t)g.nshot * std::max(P->nsource, 0) * 3);
+    if (P->nsource > 0) {
+        if (!P->source_location || !P->source_waveform) {
+            set_error("forward: null source arrays");
+            return -1;
+        }
+        DGX_TRY(hipMemcpyAsync(hsloc.data(), P->source_location, hsloc.size() * sizeof(int),
+                               hipMemcpyDeviceToHost, stream),
+                "forward: copy source locations");
+    }
+    std::vector<int> hrloc((size_t)g.nshot * std::max(P->nreceiver, 0) * 3);
+    if (P->nreceiver > 0) {
+        if (!P->receiver_location || !P->receiver_data) {
+            set_error("forward: null receiver arrays");
+            return -1;
+        }
+        DGX_TRY(hipMemcpyAsync(hrloc.data(), P->receiver_location, hrloc.size() * sizeof(int),
+                               hipMemcpyDeviceToHost, stream),
+                "forward: copy receiver locations");
+    }
+    DGX_TRY(hipStreamSynchronize(stream), "forward: stream synchronize");
+    for (size_t q = 0; q < hsloc.size(); q += 3)
+        if (!loc_ok(g, &hsloc[q])) {
+            set_error("forward: source location (%d, %d, %d) outside the field grid", hsloc[q], hsloc[q + 1],
+                      hsloc[q + 2]);
+            return -1;
+        }
+    for (size_t q = 0; q < hrloc.size(); q += 3)
+        if (!loc_ok(g, &hrloc[q])) {
+            set_error("forward: receiver location (%d, %d, %d) outside the field grid", hrloc[q], hrloc[q + 1],
+                      hrloc[q + 2]);
+            return -1;
+        }
+
+    const bool inject = P->nsource > 0 && P->source_component >= 0 && P->source_component <= 2;
+    long long nblocks = (long long)grid.x * grid.y * grid.z;
+    std::vector<int> boff((size_t)nblocks + 1, 0);
+    std::vector<int2> ents;
+    if (inject) {
+        struct E4 {
+            long long blk;
+            int order;
+            int cell;
+            int src;
+        };
+        std::vector<E4> tmp;
+        for (int s = 0; s < g.nshot; ++s)
+            for (int k = 0; k < P->nsource; ++k) {
+                const int* L = &hsloc[((size_t)s * P->nsource + k) * 3];
+                long long m = (long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2];
+                int p = L[1] * g.N[2] + L[2];
+                long long blk = (long long)(p / 256) + (long long)grid.x * (L[0] + (long long)grid.y * s);
+                tmp.push_back({blk, s * P->nsource + k, (int)m, k});
+            }
+        std::stable_sort(tmp.begin(), tmp.end(), [](const E4& a, const E4& b) { return a.blk < b.blk; });
+        for (const E4& e : tmp) {
+            boff[(size_t)e.blk + 1] += 1;
+            ents.push_back(make_int2(e.cell, e.src));
+        }
+        for (size_t b = 1; b < boff.size(); ++b) boff[b] += boff[b - 1];
+    }
+    DevBuf d_boff, d_ents;
+    SourceBlocks src{};
+    if (inject) {
+        if (!d_boff.alloc(boff.size() * sizeof(int)) || !d_ents.alloc(ents.size() * sizeof(int2))) {
+            set_error("forward: out of device memory for source lists");
+            return -1;
+        }
+        DGX_TRY(hipMemcpyAsync(d_boff.p, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice, stream),
+                "forward: upload source lists");
+        DGX_TRY(hipMemcpyAsync(d_ents.p, ents.data(), ents.size() * sizeof(int2), hipMemcpyHostToDevice, stream),
+                "forward: upload source entries");
+        src.block_off = (const int*)d_boff.p;
+        src.entries = (const int2*)d_ents.p;
+        src.count = (int)ents.size();
+        src.wave = P->source_waveform;
+        src.nt = nt;
+        src.comp = P->source_component;
+        src.dl = g.d[P->source_component];
+        src.vol = g.d[0] * g.d[1] * g.d[2];
+    }
+
+    // --- history buffers -----------------------------------------------------
+    const bool int8 = pl.kind == kInt8;
+    if (pl.history && (!P->E_saved || ((pl.store_r || pl.e_only) && !P->R_saved))) {
+        set_error("forward: null history buffer");
+        return -1;
+    }
+    DevBuf d_rscratch;  // FP32 R^n frame for INT8 encoding
+    if (pl.history && pl.store_r && int8) {
+        if (!d_rscratch.alloc((size_t)pl.comps * pl.snap * sizeof(float))) {
+            set_error("forward: out of device memory for the INT8 R frame (%lld bytes)",
+                      (long long)pl.comps * pl.snap * 4);
+            return -1;
+        }
+    }
+
+    launch_coefficients(g, P->eps_r_pad, P->sigma_pad, P->mu_r_pad, co, P->dt, stream);
+
+    StepArgs a{};
+    a.ca = co[0];
+    a.ccurl = co[1];
+    a.cb = co[2];
+    a.hcurl = co[4];
+    a.hrhs = co[5];
+    for (int c = 0; c < 3; ++c) {
+        a.E.c[c] = P->E[c];
+        a.H.c[c] = P->H[c];
+        a.Ec.c[c] = P->E[c];
+        a.Hc.c[c] = P->H[c];
+    }
+    a.cpe = pl.cpe;
+    a.cph = pl.cph;
+    a.has_cpml = pl.has_cpml;
+    a.src = src;
+    const size_t eb = elem_bytes(pl.kind);
+    const int rcomp = (P->receiver_component >= 0 && P->receiver_component <= 2) ? P->receiver_component : 2;
+
+    for (int it = 0; it < nt; ++it) {
+        const bool save = pl.history && (it % S) == 0;
+        const int ts = it / S;
+        HistWrite hw{};
+        hw.mode = pl.comps == 3 ? 3 : 2;
+        hw.box = pl.box;
+        if (save) {
+            if (int8) {
+                launch_quant_int8(g, a.Ec, nullptr, 0, int8_writer(pl, P->E_saved, pl.nt_saved, ts), stream);
+            } else {
+                hw.save_e = 1;
+                hw.e_kind = pl.kind;
+                hw.e_base = (unsigned char*)P->E_saved + (size_t)ts * pl.snap * eb;
+                hw.e_cstride = pl.cstride;
+            }
+            if (pl.store_r) {
+                hw.save_r = 1;
+                if (int8) {
+                    hw.r_kind = kF32;
+                    hw.r_base = d_rscratch.p;
+                    hw.r_cstride = pl.snap;
+                } else {
+                    hw.r_kind = pl.kind;
+                    hw.r_base = (unsigned char*)P->R_saved + (size_t)ts * pl.snap * eb;
+                    hw.r_cstride = pl.cstride;
+                }
+            }
+            hw.active = hw.save_e || hw.save_r;
+        }
+        a.hist = hw;
+        a.src.it = it;
+        launch_forward_step(pl.order, g, a, stream);
+        if (save && pl.store_r && int8)
+            launch_quant_int8(g, a.Ec, (const float*)d_rscratch.p, pl.snap,
+                              int8_writer(pl, P->R_saved, pl.nt_saved, ts), stream);
+        if (P->nreceiver > 0)
+            launch_sample(g, P->E[rcomp], P->receiver_location, P->receiver_data, P->nreceiver, nt, it, stream);
+    }
+    if (pl.e_only) {
+        if (int8) {
+            launch_quant_int8(g, a.Ec, nullptr, 0, int8_writer(pl, P->R_saved, 1, 0), stream);
+        } else {
+            HistWrite hw{};
+            hw.mode = pl.comps == 3 ? 3 : 2;
+            hw.box = pl.box;
+            hw.e_kind = pl.kind;
+            hw.e_base = P->R_saved;
+            hw.e_cstride = pl.snap;
+            launch_save_frame(g, a.Ec, hw, stream);
+        }
+    }
+    DGX_TRY(hipGetLastError(), "forward: kernel launch");
+    DGX_TRY(hipStreamSynchronize(stream), "forward: stream synchronize");
+    return 0;
+}
+
+int run_backward(const dgx_solve* P, hipStream_t stream)
+{
+    Plan pl;
+    if (make_plan(P, true, pl)) return -1;
+    const Grid& g = pl.g;
+    const int nt = P->nt;
+    const int S = P->sampling_interval;
+    float* const* co = P->update_coeff;
+    const bool want_mat = P->eps_r_requires_grad || P->sigma_requires_grad;
+    const bool want_src = P->source_requires_grad == 1 && P->nsource > 0 && P->source_component >= 0 &&
+                          P->source_component <= 2;
+    const bool fp64 = P->accumulate_fp64 != 0;
+
+    std::vector<int> hrloc((size_t)g.nshot * std::max(P->nreceiver, 0) * 3);
+    std::vector<int> hsloc;
+    if (P->nreceiver > 0) {
+        if (!P->receiver_location || !P->data_grad) {
+            set_error("backward: null receiver arrays");
+            return -1;
+        }
+        DGX_TRY(hipMemcpyAsync(hrloc.data(), P->receiver_location, hrloc.size() * sizeof(int),
+                               hipMemcpyDeviceToHost, stream),
+                "backward: copy receiver locations");
+    }
+    if (want_src) {
+        if (!P->source_location || !P->grad_source) {
+            set_error("backward: null source arrays");
+            return -1;
+        }
+        hsloc.resize((size_t)g.nshot * P->nsource * 3);
+        DGX_TRY(hipMemcpyAsync(hsloc.data(), P->source_location, hsloc.size() * sizeof(int),
+                               hipMemcpyDeviceToHost, stream),
+                "backward: copy source locations");
+    }
+    DGX_TRY(hipStreamSynchronize(stream), "backward: stream synchronize");
+    for (size_t q = 0; q < hrloc.size(); q += 3)
+        if (!loc_ok(g, &hrloc[q])) {
+            set_error("backward: receiver location (%d, %d, %d) outside the field grid", hrloc[q], hrloc[q + 1],
+                      hrloc[q + 2]);
+            return -1;
+        }
+    for (size_t q = 0; q < hsloc.size(); q += 3)
+        if (!loc_ok(g, &hsloc[q])) {
+            set_error("backward: source location (%d, %d, %d) outside the field grid", hsloc[q], hsloc[q + 1],
+                      hsloc[q + 2]);
+            return -1;
+        }
+    if (want_mat) {
+        if (!P->E_saved || !P->R_saved || !P->sigma_pad ||
+            (P->eps_r_requires_grad && !P->grad_eps_r) || (P->sigma_requires_grad && !P->grad_sigma)) {
+            set_error("backward: null history or gradient buffer");
+            return -1;
+        }
+    }
+
+    // receiver groups: one per (shot, cell), members in receiver order
+    const bool adj_rx = P->nreceiver > 0 && P->receiver_component >= 0 && P->receiver_component <= 2;
+    std::vector<int> goff(1, 0), gshot, gmem;
+    std::vector<long long> gcell;
+    if (adj_rx) {
+        for (int s = 0; s < g.nshot; ++s) {
+            std::vector<std::pair<long long, int>> cells;
+            for (int r = 0; r < P->nreceiver; ++r) {
+                const int* L = &hrloc[((size_t)s * P->nreceiver + r) * 3];
+                cells.push_back({(long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2], r});
+            }
+            std::stable_sort(cells.begin(), cells.end(),
+                             [](const std::pair<long long, int>& x, const std::pair<long long, int>& y) {
+                                 return x.first < y.first;
+                             });
+            for (size_t q = 0; q < cells.size(); ++q) {
+                if (q == 0 || cells[q].first != cells[q - 1].first) {
+                    if (q) goff.push_back((int)gmem.size());
+                    gshot.push_back(s);
+                    gcell.push_back(cells[q].first);
+                }
+                gmem.push_back(cells[q].second);
+            }
+            if (!cells.empty()) goff.push_back((int)gmem.size());
+        }
+    }
+    DevBuf d_goff, d_gshot, d_gcell, d_gmem;
+    ReceiverGroups rg{};
+    if (adj_rx && !gshot.empty()) {
+        if (!d_goff.alloc(goff.size() * 4) || !d_gshot.alloc(gshot.size() * 4) || !d_gcell.alloc(gcell.size() * 8) ||
+            !d_gmem.alloc(gmem.size() * 4)) {
+            set_error("backward: out of device memory for receiver groups");
+            return -1;
+        }
+        DGX_TRY(hipMemcpyAsync(d_goff.p, goff.data(), goff.size() * 4, hipMemcpyHostToDevice, stream), "upload");
+        DGX_TRY(hipMemcpyAsync(d_gshot.p, gshot.data(), gshot.size() * 4, hipMemcpyHostToDevice, stream), "upload");
+        DGX_TRY(hipMemcpyAsync(d_gcell.p, gcell.data(), gcell.size() * 8, hipMemcpyHostToDevice, stream), "upload");
+        DGX_TRY(hipMemcpyAsync(d_gmem.p, gmem.data(), gmem.size() * 4, hipMemcpyHostToDevice, stream), "upload");
+        rg.off = (const int*)d_goff.p;
+        rg.shot = (const int*)d_gshot.p;
+        rg.cell = (const long long*)d_gcell.p;
+        rg.member = (const int*)d_gmem.p;
+        rg.ngroups = (int)gshot.size();
+    }
+
+    // CPML weight scratch, shaped like the psi arrays
+    DevBuf d_w;
+    Cpml we{}, wh{};
+    if (pl.has_cpml) {
+        long long tot = 0;
+        for (int f = 0; f < 6; ++f)
+            for (int w = 0; w < 2; ++w) tot += psi_elems(g, true, f, w) + psi_elems(g, false, f, w);
+        if (!d_w.alloc((size_t)tot * sizeof(float))) {
+            set_error("backward: out of device memory for CPML weights");
+            return -1;
+        }
+        float* p = (float*)d_w.p;
+        for (int f = 0; f < 6; ++f)
+            for (int w = 0; w < 2; ++w) {
+                we.psi[f][w] = p;
+                p += psi_elems(g, true, f, w);
+                wh.psi[f][w] = p;
+                p += psi_elems(g, false, f, w);
+            }
+        for (int f = 0; f < 6; ++f) {
+            we.coef[f] = pl.cpe.coef[f];
+            wh.coef[f] = pl.cph.coef[f];
+        }
+    }
+
+    // FP64 accumulators of the time-summed material gradients
+    const long long ngrad = (long long)(g.N[0] - 1) * (g.N[1] - 1) * (g.N[2] - 1);
+    DevBuf d_acc;
+    void* geps = P->grad_eps_r;
+    void* gsig = P->grad_sigma;
+    if (want_mat && fp64) {
+        if (!d_acc.alloc((size_t)ngrad * 2 * sizeof(double))) {
+            set_error("backward: out of device memory for FP64 gradient accumulators");
+            return -1;
+        }
+        DGX_TRY(hipMemsetAsync(d_acc.p, 0, (size_t)ngrad * 2 * sizeof(double), stream), "backward: zero accumulators");
+        geps = d_acc.p;
+        gsig = (double*)d_acc.p + ngrad;
+    }
+
+    launch_coefficients(g, P->eps_r_pad, P->sigma_pad, P->mu_r_pad, co, P->dt, stream);
+
+    AdjArgs a{};
+    a.ca = co[0];
+    a.ccurl = co[1];
+    a.cb = co[2];
+    a.hcurl = co[4];
+    a.hrhs = co[5];
+    for (int c = 0; c < 3; ++c) {
+        a.lE.c[c] = P->E[c];
+        a.lH.c[c] = P->H[c];
+        a.lEc.c[c] = P->E[c];
+        a.lHc.c[c] = P->H[c];
+    }
+    a.cpe = pl.cpe;
+    a.cph = pl.cph;
+    a.we = we;
+    a.wh = wh;
+    a.has_cpml = pl.has_cpml;
+
+    MatGrad mg{};
+    mg.ca = co[0];
+    mg.cb = co[2];
+    mg.sig = P->sigma_pad;
+    mg.kind = pl.kind;
+    mg.e_only = pl.e_only;
+    mg.mode = pl.comps == 3 ? 3 : 2;
+    mg.want_eps = P->eps_r_requires_grad == 1;
+    mg.want_sig = P->sigma_requires_grad == 1;
+    mg.dt = P->dt;
+    mg.box = pl.box;
+    const size_t eb = elem_bytes(pl.kind);
+    const float geo = want_src ? g.d[P->source_component] / (g.d[0] * g.d[1] * g.d[2]) : 0.0f;
+
+    for (int it = nt - 1; it >= 0; --it) {
+        if (adj_rx)
+            launch_adj_receivers(g, rg, P->E[P->receiver_component], P->data_grad, P->nreceiver, nt, it, stream);
+        if (want_src)
+            launch_adj_source(g, P->source_location, co[2], P->E[P->source_component], P->grad_source, P->nsource,
+                              nt, it, geo, stream);
+        if (want_mat && it % S == 0) {
+            int wgt = S;
+            const int ts = it / S;
+            if (it + wgt > nt) wgt = nt - it;
+            mg.weight = wgt;
+            if (pl.kind == kInt8) {
+                mg.e8 = int8_view(pl, P->E_saved, pl.nt_saved, ts);
+                if (!pl.e_only) mg.s8 = int8_view(pl, P->R_saved, pl.nt_saved, ts);
+                else if (ts + 1 < pl.nt_saved) mg.s8 = int8_view(pl, P->E_saved, pl.nt_saved, ts + 1);
+                else mg.s8 = int8_view(pl, P->R_saved, 1, 0);
+            } else {
+                mg.eframe = (const unsigned char*)P->E_saved + (size_t)ts * pl.snap * eb;
+                mg.e_cstride = pl.cstride;
+                mg.s_cstride = pl.cstride;
+                if (!pl.e_only) {
+                    mg.sframe = (const unsigned char*)P->R_saved + (size_t)ts * pl.snap * eb;
+                } else if (ts + 1 < pl.nt_saved) {
+                    mg.sframe = (const unsigned char*)P->E_saved + (size_t)(ts + 1) * pl.snap * eb;
+                } else {
+                    mg.sframe = P->R_saved;
+                    mg.s_cstride = pl.snap;
+                }
+            }
+            launch_material_gradient(g, a.lEc, mg, geps, gsig, fp64, stream);
+        }
+        launch_adjoint_step(pl.order, g, a, stream);
+    }
+    if (want_mat && fp64) {
+        if (P->eps_r_requires_grad == 1) launch_add_f64((const double*)geps, P->grad_eps_r, ngrad, stream);
+        if (P->sigma_requires_grad == 1) launch_add_f64((const double*)gsig, P->grad_sigma, ngrad, stream);
+    }
+    DGX_TRY(hipGetLastError(), "backward: kernel launch");
+    DGX_TRY(hipStreamSynchronize(stream), "backward: stream synchronize");
+    return 0;
+}
+
+// --- FP16/BF16 codec self-test ----------------------------------------------
+
+__global__ void k_codec(const float* in, void* enc, float* dec, long long count, int kind, int backend)
+{
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= count) return;
+    float v = in[n];
+    uint16_t bits;
+    if (kind == kF16) {
+        if (backend == 0) {
+            bits = f32_to_f16_bits(v);
+        } else {
+            _Float16 hv = (_Float16)v;  // hardware round-to-nearest-even conversion
+            __builtin_memcpy(&bits, &hv, 2);
+        }
+        dec[n] = f16_bits_to_f32(bits);
+    } else if (kind == kBF16) {
+        if (backend == 0) {
+            bits = f32_to_bf16_bits(v);
+        } else {
+            __bf16 bv = (__bf16)v;  // hardware round-to-nearest-even conversion
+            __builtin_memcpy(&bits, &bv, 2);
+        }
+        dec[n] = bf16_bits_to_f32(bits);
+    } else {
+        ((float*)enc)[n] = v;
+        dec[n] = v;
+        return;
+    }
+    ((uint16_t*)enc)[n] = bits;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Exported C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+DGX_API int dgx_abi_version(void) { return DGX_ABI_VERSION; }
+
+DGX_API const char* dgx_last_error(void) { return g_error; }
+
+DGX_API int dgx_forward(const dgx_solve* solve, dgx_stream_t stream)
+{
+    g_error[0] = '\0';
+    return run_forward(solve, (hipStream_t)stream);
+}
+
+DGX_API int dgx_backward(const dgx_solve* solve, dgx_stream_t stream)
+{
+    g_error[0] = '\0';
+    return run_backward(solve, (hipStream_t)stream);
+}
+
+DGX_API int dgx_codec_roundtrip(const float* input, void* encoded, float* decoded, long long count,
+                                int storage_kind, int conversion_backend, dgx_stream_t stream)
+{
+    g_error[0] = '\0';
+    if (count <= 0) return 0;
+    if (!input || !encoded || !decoded) {
+        set_error("codec: null pointer");
+        return -1;
+    }
+    hipLaunchKernelGGL(k_codec, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, (hipStream_t)stream, input,
+                       encoded, decoded, count, storage_kind, conversion_backend);
+    DGX_TRY(hipGetLastError(), "codec: kernel launch");
+    DGX_TRY(hipStreamSynchronize((hipStream_t)stream), "codec: stream synchronize");
+    return 0;
+}
+
+DGX_API long long dgx_scratch_bytes(const dgx_solve* solve, int backward)
+{
+    Plan pl;
+    if (make_plan(solve, backward != 0, pl)) return -1;
+    const Grid& g = pl.g;
+    long long bytes = 0;
+    if (!backward) {
+        if (pl.history && pl.store_r && pl.kind == kInt8) bytes += (long long)pl.comps * pl.snap * 4;
+        dim3 grid = cell_grid(g);
+        bytes += ((long long)grid.x * grid.y * grid.z + 1) * 4;
+    } else {
+        if (pl.has_cpml)
+            for (int f = 0; f < 6; ++f)
+                for (int w = 0; w < 2; ++w) bytes += 4 * (psi_elems(g, true, f, w) + psi_elems(g, false, f, w));
+        if (solve->accumulate_fp64) bytes += 16LL * (g.N[0] - 1) * (g.N[1] - 1) * (g.N[2] - 1);
+    }
+    return bytes;
+}
+
+// --- reference ABI-6 export set ----------------------------------------------
+
+DGX_API int deepgpr_supports_error_reporting(void) { return 1; }
+DGX_API const char* deepgpr_last_error(void) { return g_error; }
+DGX_API void deepgpr_clear_last_error(void) { g_error[0] = '\0'; }
+DGX_API void deepgpr_clear_cuda_resource_cache(void) {}
+DGX_API int deepgpr_deterministic_adjoint(void) { return 1; }
+DGX_API int deepgpr_abi_version(void) { return DEEPGPR_ABI_VERSION; }
+DGX_API int deepgpr_supports_external_pml(void) { return 1; }
+DGX_API int deepgpr_supports_pml_material_gradients(void) { return 1; }
+DGX_API int deepgpr_supports_int8_wavefield(void) { return 1; }
+DGX_API int deepgpr_supports_conversion_backends(void) { return 1; }
+DGX_API int deepgpr_supports_int8_reduction_backends(void) { return 1; }
+DGX_API int deepgpr_supports_rhs_reconstruction(void) { return 1; }
+DGX_API int deepgpr_supports_physical_history(void) { return 1; }
+DGX_API int deepgpr_supports_tm2d_fast_path(void) { return 1; }
+
+DGX_API void deepgpr_test_wavefield_conversion(const float* input, void* encoded, float* decoded, long long count,
+                                               int storage_kind, int conversion_backend)
+{
+    (void)dgx_codec_roundtrip(input, encoded, decoded, count, storage_kind, conversion_backend, nullptr);
+}
+
+DGX_API void set_fdtd_order(int order) { g_abi6_order = (order == 4 || order == 8) ? order : 2; }
+
+DGX_API void forward(
+    const float* eps_r_pad, const float* sigma_pad, const float* mu_r_pad, void* E_saved, void* R_saved,
+    float* Ex, float* Ey, float* Ez, float* Hx, float* Hy, float* Hz, float* ce_hist, float* ce_curl,
+    float* ce_rhs, float* ch_hist, float* ch_curl, float* ch_rhs, float* x0_e_phi1, float* x0_e_phi2,
+    float* x0_h_phi1, float* x0_h_phi2, float* xm_e_phi1, float* xm_e_phi2, float* xm_h_phi1, float* xm_h_phi2,
+    float* y0_e_phi1, float* y0_e_phi2, float* y0_h_phi1, float* y0_h_phi2, float* ym_e_phi1, float* ym_e_phi2,
+    float* ym_h_phi1, float* ym_h_phi2, float* z0_e_phi1, float* z0_e_phi2, float* z0_h_phi1, float* z0_h_phi2,
+    float* zm_e_phi1, float* zm_e_phi2, float* zm_h_phi1, float* zm_h_phi2, int pml_x0, int pml_xm, int pml_y0,
+    int pml_ym, int pml_z0, int pml_zm, const float* x0_e_coeff, const float* xm_e_coeff, const float* y0_e_coeff,
+    const float* ym_e_coeff, const float* z0_e_coeff, const float* zm_e_coeff, const float* x0_h_coeff,
+    const float* xm_h_coeff, const float* y0_h_coeff, const float* ym_h_coeff, const float* z0_h_coeff,
+    const float* zm_h_coeff, float dt, int nt, int nshot, int nreceiver, float dx, float dy, float dz,
+    const int* receiver_location, float* receiver_data, int receiver_component, int nx_fields, int ny_fields,
+    int nz_fields, int nsource, const int* source_location, const float* source_waveform, int source_component,
+    int sampling_interval, int fwi_mode, int storage_type, int save_model_history, int use_async_offload)
+{
+    dgx_solve s;
+    std::memset(&s, 0, sizeof(s));
+    s.abi_version = DGX_ABI_VERSION;
+    s.fdtd_order = g_abi6_order;
+    s.eps_r_pad = eps_r_pad;
+    s.sigma_pad = sigma_pad;
+    s.mu_r_pad = mu_r_pad;
+    s.E_saved = E_saved;
+    s.R_saved = R_saved;
+    float* E[3] = {Ex, Ey, Ez};
+    float* H[3] = {Hx, Hy, Hz};
+    for (int c = 0; c < 3; ++c) {
+        s.E[c] = E[c];
+        s.H[c] = H[c];
+    }
+    float* uc[6] = {ce_hist, ce_curl, ce_rhs, ch_hist, ch_curl, ch_rhs};
+    for (int c = 0; c < 6; ++c) s.update_coeff[c] = uc[c];
+    float* phi[24] = {x0_e_phi1, x0_e_phi2, x0_h_phi1, x0_h_phi2, xm_e_phi1, xm_e_phi2, xm_h_phi1, xm_h_phi2,
+                      y0_e_phi1, y0_e_phi2, y0_h_phi1, y0_h_phi2, ym_e_phi1, ym_e_phi2, ym_h_phi1, ym_h_phi2,
+                      z0_e_phi1, z0_e_phi2, z0_h_phi1, z0_h_phi2, zm_e_phi1, zm_e_phi2, zm_h_phi1, zm_h_phi2};
+    for (int c = 0; c < 24; ++c) s.phi[c] = phi[c];
+    const float* pc[12] = {x0_e_coeff, xm_e_coeff, y0_e_coeff, ym_e_coeff, z0_e_coeff, zm_e_coeff,
+                           x0_h_coeff, xm_h_coeff, y0_h_coeff, ym_h_coeff, z0_h_coeff, zm_h_coeff};
+    for (int c = 0; c < 12; ++c) s.pml_coeff[c] = pc[c];
+    int pml[6] = {pml_x0, pml_xm, pml_y0, pml_ym, pml_z0, pml_zm};
+    for (int f = 0; f < 6; ++f) s.pml[f] = pml[f];
+    s.dt = dt;
+    s.nt = nt;
+    s.nshot = nshot;
+    s.nreceiver = nreceiver;
+    s.dx = dx;
+    s.dy = dy;
+    s.dz = dz;
+    s.receiver_location = receiver_location;
+    s.receiver_data = receiver_data;
+    s.receiver_component = receiver_component;
+    s.nx_fields = nx_fields;
+    s.ny_fields = ny_fields;
+    s.nz_fields = nz_fields;
+    s.nsource = nsource;
+    s.source_location = source_location;
+    s.source_waveform = source_waveform;
+    s.source_component = source_component;
+    s.sampling_interval = sampling_interval;
+    s.fwi_mode = fwi_mode;
+    s.storage_type = storage_type;
+    s.save_model_history = save_model_history;
+    s.use_async_offload = use_async_offload;
+    (void)dgx_forward(&s, nullptr);
+}
+
+DGX_API void backward(
+    const float* eps_r_pad, const float* sigma_pad, const float* mu_r_pad, const void* E_saved, const void* R_saved,
+    float* lambda_ex, float* lambda_ey, float* lambda_ez, float* lambda_hx, float* lambda_hy, float* lambda_hz,
+    float* ce_hist, float* ce_curl, float* ce_rhs, float* ch_hist, float* ch_curl, float* ch_rhs,
+    float* l_x0_e_phi1, float* l_x0_e_phi2, float* l_x0_h_phi1, float* l_x0_h_phi2, float* l_xm_e_phi1,
+    float* l_xm_e_phi2, float* l_xm_h_phi1, float* l_xm_h_phi2, float* l_y0_e_phi1, float* l_y0_e_phi2,
+    float* l_y0_h_phi1, float* l_y0_h_phi2, float* l_ym_e_phi1, float* l_ym_e_phi2, float* l_ym_h_phi1,
+    float* l_ym_h_phi2, float* l_z0_e_phi1, float* l_z0_e_phi2, float* l_z0_h_phi1, float* l_z0_h_phi2,
+    float* l_zm_e_phi1, float* l_zm_e_phi2, float* l_zm_h_phi1, float* l_zm_h_phi2, int pml_x0, int pml_xm,
+    int pml_y0, int pml_ym, int pml_z0, int pml_zm, const float* x0_e_coeff, const float* xm_e_coeff,
+    const float* y0_e_coeff, const float* ym_e_coeff, const float* z0_e_coeff, const float* zm_e_coeff,
+    const float* x0_h_coeff, const float* xm_h_coeff, const float* y0_h_coeff, const float* ym_h_coeff,
+    const float* z0_h_coeff, const float* zm_h_coeff, float dt, int nt, int nshot, int nreceiver, float dx,
+    float dy, float dz, int nx_fields, int ny_fields, int nz_fields, int ndata_source, const int* receiver_location,
+    const float* data_grad, int receiver_component, int nsource, const int* source_location, int source_component,
+    float* grad_source, int source_requires_grad, float* grad_eps_r, float* grad_sigma, int eps_r_requires_grad,
+    int sigma_requires_grad, int sampling_interval, int fwi_mode, int storage_type, int use_async_offload)
+{
+    (void)nreceiver;
+    dgx_solve s;
+    std::memset(&s, 0, sizeof(s));
+    s.abi_version = DGX_ABI_VERSION;
+    s.fdtd_order = g_abi6_order;
+    s.eps_r_pad = eps_r_pad;
+    s.sigma_pad = sigma_pad;
+    s.mu_r_pad = mu_r_pad;
+    s.E_saved = (void*)E_saved;
+    s.R_saved = (void*)R_saved;
+    float* E[3] = {lambda_ex, lambda_ey, lambda_ez};
+    float* H[3] = {lambda_hx, lambda_hy, lambda_hz};
+    for (int c = 0; c < 3; ++c) {
+        s.E[c] = E[c];
+        s.H[c] = H[c];
+    }
+    float* uc[6] = {ce_hist, ce_curl, ce_rhs, ch_hist, ch_curl, ch_rhs};
+    for (int c = 0; c < 6; ++c) s.update_coeff[c] = uc[c];
+    float* phi[24] = {l_x0_e_phi1, l_x0_e_phi2, l_x0_h_phi1, l_x0_h_phi2, l_xm_e_phi1, l_xm_e_phi2,
+                      l_xm_h_phi1, l_xm_h_phi2, l_y0_e_phi1, l_y0_e_phi2, l_y0_h_phi1, l_y0_h_phi2,
+                      l_ym_e_phi1, l_ym_e_phi2, l_ym_h_phi1, l_ym_h_phi2, l_z0_e_phi1, l_z0_e_phi2,
+                      l_z0_h_phi1, l_z0_h_phi2, l_zm_e_phi1, l_zm_e_phi2, l_zm_h_phi1, l_zm_h_phi2};
+    for (int c = 0; c < 24; ++c) s.phi[c] = phi[c];
+    const float* pc[12] = {x0_e_coeff, xm_e_coeff, y0_e_coeff, ym_e_coeff, z0_e_coeff, zm_e_coeff,
+                           x0_h_coeff, xm_h_coeff, y0_h_coeff, ym_h_coeff, z0_h_coeff, zm_h_coeff};
+    for (int c = 0; c < 12; ++c) s.pml_coeff[c] = pc[c];
+    int pml[6] = {pml_x0, pml_xm, pml_y0, pml_ym, pml_z0, pml_zm};
+    for (int f = 0; f < 6; ++f) s.pml[f] = pml[f];
+    s.dt = dt;
+    s.nt = nt;
+    s.nshot = nshot;
+    s.nreceiver = ndata_source;
+    s.dx = dx;
+    s.dy = dy;
+    s.dz = dz;
+    s.nx_fields = nx_fields;
+    s.ny_fields = ny_fields;
+    s.nz_fields = nz_fields;
+    s.receiver_location = receiver_location;
+    s.data_grad = data_grad;
+    s.receiver_component = receiver_component;
+    s.nsource = nsource;
+    s.source_location = source_location;
+    s.source_component = source_component;
+    s.grad_source = grad_source;
+    s.source_requires_grad = source_requires_grad;
+    s.grad_eps_r = grad_eps_r;
+    s.grad_sigma = grad_sigma;
+    s.eps_r_requires_grad = eps_r_requires_grad;
+    s.sigma_requires_grad = sigma_requires_grad;
+    s.sampling_interval = sampling_interval;
+    s.fwi_mode = fwi_mode;
+    s.storage_type = storage_type;
+    s.use_async_offload = use_async_offload;
+    (void)dgx_backward(&s, nullptr);
+}
+
+}  // extern "C"

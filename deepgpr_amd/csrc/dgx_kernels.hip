@@ -1,0 +1,758 @@
+// HIP kernels of the FDTD hot path for MI355X (gfx950).
+//
+// General path: one thread per Yee cell, one kernel per half step, CPML fused
+// into the curl sweep, source injection and history encoding fused into the
+// electric sweep. The adjoint is a deterministic gather: every target cell
+// re-evaluates the transposed stencil weights of its neighbours, so no
+// atomics are used anywhere and results do not depend on scheduling.
+#include "dgx_device.h"
+#include "dgx_internal.h"
+
+namespace dgx {
+
+// ---------------------------------------------------------------------------
+// Update coefficients (deepgpr_cpu.c:547-584)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_coefficients(
+    Grid g, const float* __restrict__ eps, const float* __restrict__ sig, const float* __restrict__ mu,
+    float* ca, float* ccurl, float* cb, float* hh, float* hcurl, float* hrhs, float dt)
+{
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.cells) return;
+    int i = (int)(n / g.st0);
+    int rem = (int)(n - (long long)i * g.st0);
+    int j = rem / g.N[2];
+    int k = rem - j * g.N[2];
+    if (!(i < g.N[0] - 1 && j < g.N[1] - 1 && k < g.N[2] - 1)) return;
+    float ha = kMu0 * mu[n] / dt;
+    hh[n] = 1.0f;
+    hcurl[n] = (1.0f / g.d[0]) / ha;
+    hrhs[n] = 1.0f / ha;
+    if (sig[n] > kPecSigma) {
+        ca[n] = 0.0f;
+        ccurl[n] = 0.0f;
+        cb[n] = 0.0f;
+    } else {
+        float et = kEps0 * eps[n] / dt;
+        float st = 0.5f * sig[n];
+        float ea = et + st;
+        float eb = et - st;
+        ca[n] = eb / ea;
+        ccurl[n] = (1.0f / g.d[0]) / ea;
+        cb[n] = 1.0f / ea;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward magnetic half step: curl update + H-CPML (deepgpr_cpu.c:886-1100)
+// ---------------------------------------------------------------------------
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_fwd_h(Grid g, const float* __restrict__ hcurl,
+                                               const float* __restrict__ hrhs, CF3 E, F3 H, Cpml cp,
+                                               int has_cpml)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.plane) return;
+    const int i = blockIdx.y, s = blockIdx.z;
+    const int j = p / g.N[2];
+    const int k = p - j * g.N[2];
+    const int c[3] = {i, j, k};
+    const long long m = (long long)i * g.st0 + p;
+    const long long n = (long long)s * g.cells + m;
+    const long long st[3] = {g.st0, (long long)g.N[2], 1};
+    const bool up0 = h_updated(g, 0, c), up1 = h_updated(g, 1, c), up2 = h_updated(g, 2, c);
+    const bool pml = has_cpml && in_any_face(g, false, c);
+    if (!(up0 || up1 || up2 || pml)) return;
+
+    float h[3] = {H.c[0][n], H.c[1][n], H.c[2][n]};
+    const float u1 = hcurl[m];
+    const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
+    if (up0) {
+        float dEz = dfwd<ORDER>(E.c[2], n, st[1], j, g.N[1]);
+        float dEy = dfwd<ORDER>(E.c[1], n, 1, k, g.N[2]);
+        h[0] = h[0] - uy * dEz + uz * dEy;
+    }
+    if (up1) {
+        float dEx = dfwd<ORDER>(E.c[0], n, 1, k, g.N[2]);
+        float dEz = dfwd<ORDER>(E.c[2], n, st[0], i, g.N[0]);
+        h[1] = h[1] - uz * dEx + u1 * dEz;
+    }
+    if (up2) {
+        float dEy = dfwd<ORDER>(E.c[1], n, st[0], i, g.N[0]);
+        float dEx = dfwd<ORDER>(E.c[0], n, st[1], j, g.N[1]);
+        h[2] = h[2] - u1 * dEy + uy * dEx;
+    }
+    if (pml) {
+        const float upd = hrhs[m];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int axis = f >> 1;
+            const int q = face_q(g, false, f, c[axis]);
+            if (q < 0) continue;
+            const float* R = cp.coef[f];
+            const int pp = g.pml[f];
+            const float ra1 = R[q] - 1.0f, rb = R[pp + q], re = R[2 * pp + q], rf = R[3 * pp + q];
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                int dims[3];
+                psi_dims(g, false, f, w, dims);
+                const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
+                if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
+                const int t = face_target(axis, w), sc = face_source(axis, w);
+                const float der = dfwd<ORDER>(E.c[sc], n, st[axis], c[axis], g.N[axis]) / g.d[axis];
+                float* psi = cp.psi[f][w];
+                const long long pi = psi_index(dims, s, c, axis, q);
+                const float phi = psi[pi];
+                const float corr = upd * (ra1 * der + rb * phi);
+                if (h_face_sign(axis, w) < 0.0f) h[t] -= corr;
+                else h[t] += corr;
+                psi[pi] = re * phi - rf * der;
+            }
+        }
+    }
+    if (up0 || pml) H.c[0][n] = h[0];
+    if (up1 || pml) H.c[1][n] = h[1];
+    if (up2 || pml) H.c[2][n] = h[2];
+}
+
+// ---------------------------------------------------------------------------
+// Forward electric half step: curl update + E-CPML + sources + history
+// (deepgpr_cpu.c:670-883, 644-668, 1559-1633)
+// ---------------------------------------------------------------------------
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__ ca,
+                                               const float* __restrict__ ccurl,
+                                               const float* __restrict__ cb, CF3 Hf, F3 E, Cpml cp,
+                                               int has_cpml, SourceBlocks src, HistWrite hist)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.plane) return;
+    const int i = blockIdx.y, s = blockIdx.z;
+    const int j = p / g.N[2];
+    const int k = p - j * g.N[2];
+    const int c[3] = {i, j, k};
+    const long long m = (long long)i * g.st0 + p;
+    const long long n = (long long)s * g.cells + m;
+    const long long st[3] = {g.st0, (long long)g.N[2], 1};
+    const bool up[3] = {e_updated(g, 0, c), e_updated(g, 1, c), e_updated(g, 2, c)};
+    const bool pml = has_cpml && in_any_face(g, true, c);
+
+    // block-local source entries (sorted by block, then source order)
+    const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    int sb = 0, se = 0;
+    if (src.count > 0) {
+        sb = src.block_off[blin];
+        se = src.block_off[blin + 1];
+    }
+    const bool in_box = hist.active && i >= hist.box.x0 && i < hist.box.x0 + hist.box.nx && j >= hist.box.y0 &&
+                        j < hist.box.y0 + hist.box.ny && k >= hist.box.z0 && k < hist.box.z0 + hist.box.nz;
+    if (!(up[0] || up[1] || up[2] || pml || se > sb || in_box)) return;
+
+    float e[3] = {E.c[0][n], E.c[1][n], E.c[2][n]};
+    const float eold[3] = {e[0], e[1], e[2]};
+    const float u0 = ca[m], u1 = ccurl[m];
+    const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
+    if (up[0]) {
+        float dHz = dbwd<ORDER>(Hf.c[2], n, st[1], j, g.N[1]);
+        float dHy = dbwd<ORDER>(Hf.c[1], n, 1, k, g.N[2]);
+        e[0] = u0 * e[0] + uy * dHz - uz * dHy;
+    }
+    if (up[1]) {
+        float dHx = dbwd<ORDER>(Hf.c[0], n, 1, k, g.N[2]);
+        float dHz = dbwd<ORDER>(Hf.c[2], n, st[0], i, g.N[0]);
+        e[1] = u0 * e[1] + uz * dHx - u1 * dHz;
+    }
+    if (up[2]) {
+        float dHy = dbwd<ORDER>(Hf.c[1], n, st[0], i, g.N[0]);
+        float dHx = dbwd<ORDER>(Hf.c[0], n, st[1], j, g.N[1]);
+        e[2] = u0 * e[2] + u1 * dHy - uy * dHx;
+    }
+    const float cbv = cb[m];
+    if (pml) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int axis = f >> 1;
+            const int q = face_q(g, true, f, c[axis]);
+            if (q < 0) continue;
+            const float* R = cp.coef[f];
+            const int pp = g.pml[f];
+            const float ra1 = R[q] - 1.0f, rb = R[pp + q], re = R[2 * pp + q], rf = R[3 * pp + q];
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                int dims[3];
+                psi_dims(g, true, f, w, dims);
+                const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
+                if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
+                const int t = face_target(axis, w), sc = face_source(axis, w);
+                const float der = dbwd<ORDER>(Hf.c[sc], n, st[axis], c[axis], g.N[axis]) / g.d[axis];
+                float* psi = cp.psi[f][w];
+                const long long pi = psi_index(dims, s, c, axis, q);
+                const float phi = psi[pi];
+                const float corr = cbv * (ra1 * der + rb * phi);
+                if (e_face_sign(axis, w) < 0.0f) e[t] -= corr;
+                else e[t] += corr;
+                psi[pi] = re * phi - rf * der;
+            }
+        }
+    }
+    // Hertzian dipole sources of this block, applied in source order.
+    for (int q = sb; q < se; ++q) {
+        const int2 ent = src.entries[q];
+        if ((long long)ent.x == m) {
+            const float scale = src.wave[(long long)ent.y * src.nt + src.it] * src.dl / src.vol;
+            e[src.comp] -= cbv * scale;
+        }
+    }
+    // history: E^n (pre-update) and the effective right-hand side R^n
+    if (in_box) {
+        const long long h = ((long long)(i - hist.box.x0) * hist.box.ny + (j - hist.box.y0)) * hist.box.nz +
+                            (k - hist.box.z0);
+        const long long base = (long long)s * hist.box.cells + h;
+        const int comps = hist.mode == 3 ? 3 : 1;
+        for (int ci = 0; ci < comps; ++ci) {
+            const int comp = hist.mode == 3 ? ci : 2;
+            if (hist.save_e) store_hist(hist.e_base, ci * hist.e_cstride + base, eold[comp], hist.e_kind);
+            if (hist.save_r) {
+                const float a = up[comp] ? u0 : 1.0f;
+                const float rhs = cbv != 0.0f ? (e[comp] - a * eold[comp]) / cbv : 0.0f;
+                store_hist(hist.r_base, ci * hist.r_cstride + base, rhs, hist.r_kind);
+            }
+        }
+    }
+    const bool touched = pml || se > sb;
+    if (up[0] || touched) E.c[0][n] = e[0];
+    if (up[1] || touched) E.c[1][n] = e[1];
+    if (up[2] || touched) E.c[2][n] = e[2];
+}
+
+// Receiver gather: rx[s, it, r] = E_comp at the receiver (deepgpr_cpu.c:600-623)
+__global__ __launch_bounds__(256) void k_sample(Grid g, const float* __restrict__ f, const int* __restrict__ loc,
+                                                float* __restrict__ out, int nrx, int nt, int it)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nshot * nrx) return;
+    int s = idx / nrx, r = idx - s * nrx;
+    const int* L = loc + (long long)idx * 3;
+    long long m = (long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2];
+    out[((long long)s * nt + it) * nrx + r] = f[(long long)s * g.cells + m];
+}
+
+// Copy E components of the history box into one frame (final E^nt frame).
+__global__ __launch_bounds__(256) void k_save_frame(Grid g, CF3 E, HistWrite hist)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)g.nshot * hist.box.cells) return;
+    int s = (int)(idx / hist.box.cells);
+    long long h = idx - (long long)s * hist.box.cells;
+    long long x = h / ((long long)hist.box.ny * hist.box.nz) + hist.box.x0;
+    long long y = (h / hist.box.nz) % hist.box.ny + hist.box.y0;
+    long long z = h % hist.box.nz + hist.box.z0;
+    long long n = (long long)s * g.cells + x * g.st0 + y * g.N[2] + z;
+    int comps = hist.mode == 3 ? 3 : 1;
+    for (int ci = 0; ci < comps; ++ci) {
+        int comp = hist.mode == 3 ? ci : 2;
+        store_hist(hist.e_base, ci * hist.e_cstride + idx, E.c[comp][n], hist.e_kind);
+    }
+}
+
+// One workgroup per INT8 tile: max |v| over the tile, scale = max/127 (1 when
+// max < 1e-30), q = rint(v/scale) clamped to +-127 (deepgpr.cu:2885-3030).
+// Values come from the field arrays (box-mapped) or from a box-ordered FP32
+// frame (comp stride src_cstride).
+__global__ __launch_bounds__(256) void k_quant_int8(Grid g, CF3 E, const float* __restrict__ src,
+                                                    long long src_cstride, Int8Write w)
+{
+    __shared__ float red[256];
+    const int tile = blockIdx.x, s = blockIdx.y, ci = blockIdx.z;
+    const int comp = w.mode == 3 ? ci : 2;
+    const int tz = tile % w.nbz, ty = (tile / w.nbz) % w.nby, tx = tile / (w.nbz * w.nby);
+    const int local = threadIdx.x;
+    const int lx = local / (w.by * w.bz), ly = (local / w.bz) % w.by, lz = local % w.bz;
+    const int x = tx * w.bx + lx, y = ty * w.by + ly, z = tz * w.bz + lz;
+    const bool valid = x < w.box.nx && y < w.box.ny && z < w.box.nz;
+    const long long h = valid ? ((long long)x * w.box.ny + y) * w.box.nz + z : 0;
+    float v = 0.0f;
+    if (valid) {
+        if (src) {
+            v = src[ci * src_cstride + (long long)s * w.box.cells + h];
+        } else {
+            long long n = (long long)s * g.cells + (long long)(x + w.box.x0) * g.st0 +
+                          (long long)(y + w.box.y0) * g.N[2] + (z + w.box.z0);
+            v = E.c[comp][n];
+        }
+    }
+    red[local] = v < 0.0f ? -v : v;
+    __syncthreads();
+    for (int stride = blockDim.x >> 1; stride > 0; stride >>= 1) {
+        if (local < stride && red[local + stride] > red[local]) red[local] = red[local + stride];
+        __syncthreads();
+    }
+    const float peak = red[0];
+    const float scale = peak >= 1.0e-30f ? peak / 127.0f : 1.0f;
+    const long long snap = (long long)g.nshot * w.box.cells;
+    if (local == 0) w.scale[((long long)ci * w.frames + w.frame) * g.nshot * w.tiles + (long long)s * w.tiles + tile] = scale;
+    if (valid) {
+        float r = rintf(v / scale);
+        r = fminf(fmaxf(r, -127.0f), 127.0f);
+        w.q[((long long)ci * w.frames + w.frame) * snap + (long long)s * w.box.cells + h] = (signed char)(int)r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint: receivers^T and source^T (deepgpr_cpu.c:1479-1534)
+// ---------------------------------------------------------------------------
+
+// One thread per (shot, receiver cell): coincident receivers add in index order.
+__global__ __launch_bounds__(256) void k_adj_receivers(Grid g, ReceiverGroups rg, float* __restrict__ lam,
+                                                       const float* __restrict__ dgrad, int nrx, int nt, int it)
+{
+    int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= rg.ngroups) return;
+    int s = rg.shot[gi];
+    long long n = (long long)s * g.cells + rg.cell[gi];
+    float acc = lam[n];
+    for (int q = rg.off[gi]; q < rg.off[gi + 1]; ++q)
+        acc += dgrad[((long long)s * nt + it) * nrx + rg.member[q]];
+    lam[n] = acc;
+}
+
+// One thread per source: shots summed in order.
+__global__ __launch_bounds__(64) void k_adj_source(Grid g, const int* __restrict__ loc, const float* __restrict__ cb,
+                                                   const float* __restrict__ lam, float* __restrict__ gsrc,
+                                                   int nsrc, int nt, int it, float geo)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsrc) return;
+    float acc = gsrc[(long long)k * nt + it];
+    for (int s = 0; s < g.nshot; ++s) {
+        const int* L = loc + ((long long)s * nsrc + k) * 3;
+        long long m = (long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2];
+        acc += -cb[m] * geo * lam[(long long)s * g.cells + m];
+    }
+    gsrc[(long long)k * nt + it] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Material gradient fused with the history decode (deepgpr_cpu.c:1659-1745)
+// ---------------------------------------------------------------------------
+template <typename ACC>
+__global__ __launch_bounds__(256) void k_material_gradient(Grid g, CF3 L, MatGrad mg, ACC* __restrict__ geps,
+                                                           ACC* __restrict__ gsig)
+{
+    long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const Box b = mg.box;
+    if (h >= b.cells) return;
+    const int ix = (int)(h / ((long long)b.ny * b.nz)) + b.x0;
+    const int iy = (int)((h / b.nz) % b.ny) + b.y0;
+    const int iz = (int)(h % b.nz) + b.z0;
+    const int c[3] = {ix, iy, iz};
+    const long long m = (long long)ix * g.st0 + (long long)iy * g.N[2] + iz;
+    const long long out = ((long long)ix * (g.N[1] - 1) + iy) * (g.N[2] - 1) + iz;
+    const float cav = mg.ca[m], cbv = mg.cb[m];
+    const bool active = mg.sig[m] <= kPecSigma;
+    const int comps = mg.mode == 3 ? 3 : 1;
+    const float wgt = (float)mg.weight;
+    ACC ge = 0, gs = 0;
+    for (int s = 0; s < g.nshot; ++s) {
+        const long long f = (long long)s * g.cells + m;
+        const long long base = (long long)s * b.cells + h;
+        for (int ci = 0; ci < comps; ++ci) {
+            const int comp = mg.mode == 3 ? ci : 2;
+            const bool up = e_updated(g, comp, c);
+            float e_old, sec;
+            if (mg.kind == kInt8) {
+                e_old = int8_load(mg.e8, b, ci, s, h);
+                sec = int8_load(mg.s8, b, ci, s, h);
+            } else {
+                e_old = load_hist(mg.eframe, ci * mg.e_cstride + base, mg.kind);
+                sec = load_hist(mg.sframe, ci * mg.s_cstride + base, mg.kind);
+            }
+            float lam = L.c[comp][f];
+            float rhs = sec;
+            if (mg.e_only) {
+                const float a = up ? cav : 1.0f;
+                rhs = cbv != 0.0f ? (sec - a * e_old) / cbv : 0.0f;
+            }
+            if (e_old == 0.0f && rhs == 0.0f) lam = 0.0f;
+            const float gca = up ? lam * e_old * wgt : 0.0f;
+            const float gcb = lam * rhs * wgt;
+            if (active) {
+                if (mg.want_eps) {
+                    const float dca = kEps0 * (1.0f - cav) * cbv / mg.dt;
+                    const float dcb = -kEps0 * cbv * cbv / mg.dt;
+                    ge += (ACC)(gca * dca + gcb * dcb);
+                }
+                if (mg.want_sig) {
+                    const float dca = -0.5f * (1.0f + cav) * cbv;
+                    const float dcb = -0.5f * cbv * cbv;
+                    gs += (ACC)(gca * dca + gcb * dcb);
+                }
+            }
+        }
+    }
+    if (mg.want_eps) geps[out] += ge;
+    if (mg.want_sig) gsig[out] += gs;
+}
+
+__global__ __launch_bounds__(256) void k_add_f64_to_f32(const double* __restrict__ a, float* __restrict__ out,
+                                                        long long count)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < count) out[idx] += (float)a[idx];
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint CPML weights: (E-CPML)^T or (H-CPML)^T of one half step
+// (deepgpr_cpu.c:509-527, 1153-1239, 1332-1410)
+// ---------------------------------------------------------------------------
+// Each face cell advances its auxiliary cotangent and stores the weight its
+// CPML term contributes to the transposed derivative, in an array shaped like
+// the auxiliary array. The gather kernels read those weights.
+template <bool ELECTRIC>
+__global__ __launch_bounds__(256) void k_adj_pml(Grid g, const float* __restrict__ upd, CF3 lam, Cpml cp,
+                                                 Cpml wout)
+{
+    const int f = blockIdx.y, s = blockIdx.z;
+    const int pp = g.pml[f];
+    if (pp <= 0) return;
+    const int axis = f >> 1;
+    const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
+    const int n_face = pp;  // cells of the face region along its axis
+    const long long plane = (long long)g.N[a1] * g.N[a2];
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= plane * n_face) return;
+    const int rf_idx = (int)(idx / plane);
+    const long long rem = idx - (long long)rf_idx * plane;
+    const int ca1 = (int)(rem / g.N[a2]);
+    const int ca2 = (int)(rem - (long long)ca1 * g.N[a2]);
+    const int n_ax = g.N[axis];
+    int cax;
+    if ((f & 1) == 0) cax = ELECTRIC ? 1 + rf_idx : rf_idx;
+    else cax = n_ax - 1 - pp + rf_idx;
+    int c[3];
+    c[axis] = cax;
+    c[a1] = ca1;
+    c[a2] = ca2;
+    const int q = face_q(g, ELECTRIC, f, cax);
+    if (q < 0) return;
+    const long long m = (long long)c[0] * g.st0 + (long long)c[1] * g.N[2] + c[2];
+    const long long n = (long long)s * g.cells + m;
+    const float* R = cp.coef[f];
+    const float ra1 = R[q] - 1.0f, rb = R[pp + q], re = R[2 * pp + q], rf = R[3 * pp + q];
+    const float u = upd[m];
+    const float inv = 1.0f / g.d[axis];
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        int dims[3];
+        psi_dims(g, ELECTRIC, f, w, dims);
+        if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
+        const int t = face_target(axis, w);
+        const float sign = ELECTRIC ? e_face_sign(axis, w) : h_face_sign(axis, w);
+        const long long pi = psi_index(dims, s, c, axis, q);
+        const float lt = lam.c[t][n];
+        const float old = cp.psi[f][w][pi];
+        wout.psi[f][w][pi] = (sign * u * ra1 * lt - rf * old) * inv;
+        cp.psi[f][w][pi] = sign * u * rb * lt + re * old;
+    }
+}
+
+// Weight of transposed-stencil channel (src component u, axis a) at cell q.
+// ELECTRIC: channel of (E update + E-CPML)^T, built from lambda_E and ce_curl;
+// otherwise of (H update + H-CPML)^T from lambda_H and ch_curl.
+template <bool ELECTRIC>
+__device__ __forceinline__ float channel_weight(const Grid& g, const CF3& lam, const float* __restrict__ curl,
+                                                const Cpml& wts, bool cpml, int s, int u, int a, const int q[3])
+{
+    float acc = 0.0f;
+    const long long m = (long long)q[0] * g.st0 + (long long)q[1] * g.N[2] + q[2];
+    if (cpml) {
+        const int w = face_source(a, 0) == u ? 0 : 1;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int f = 2 * a + side;
+            const int qq = face_q(g, ELECTRIC, f, q[a]);
+            if (qq < 0) continue;
+            int dims[3];
+            psi_dims(g, ELECTRIC, f, w, dims);
+            const int a1 = a == 2 ? 0 : a + 1, a2 = a == 0 ? 2 : a - 1;
+            if (q[a1] >= dims[a1] || q[a2] >= dims[a2]) continue;
+            acc += wts.psi[f][w][psi_index(dims, s, q, a, qq)];
+        }
+    }
+    const int t = 3 - u - a;  // component whose curl term differentiates u along a
+    const bool up = ELECTRIC ? e_updated(g, t, q) : h_updated(g, t, q);
+    if (up) {
+        const float k = axis_coeff(g, curl[m], a);
+        const int t1 = t == 2 ? 0 : t + 1;
+        // E: channel (t+2, t+1) gets +k, (t+1, t+2) gets -k; H the opposite.
+        const bool plus = (a == t1) == ELECTRIC;
+        const float lt = lam.c[t][(long long)s * g.cells + m];
+        acc += (plus ? k : -k) * lt;
+    }
+    return acc;
+}
+
+// Transposes of the staggered differences (deepgpr_cpu.c:430-511).
+template <int ORDER, typename WF>
+__device__ __forceinline__ float gather_bwd_T(WF wf, int c, int n)
+{
+    if constexpr (ORDER <= 2) {
+        float acc = wf(0);
+        if (c + 1 < n) acc -= wf(1);
+        return acc;
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float acc = 0.0f;
+        if (c >= 2 * R - 1 && c + 2 * R - 1 < n) {
+#pragma unroll
+            for (int r = 1; r <= R; ++r) {
+                const float kc = stencil_c(R, r);
+                acc += kc * wf(-(r - 1));
+                acc -= kc * wf(r);
+            }
+            return acc;
+        }
+        for (int r = 1; r <= R; ++r) {
+            const int sp = c - (r - 1);
+            if (sp >= 0) {
+                const int rs = fit_bwd(sp, n, R);
+                if (r <= rs) acc += stencil_c(rs, r) * wf(-(r - 1));
+            }
+            const int sm = c + r;
+            if (sm < n) {
+                const int rs = fit_bwd(sm, n, R);
+                if (r <= rs) acc -= stencil_c(rs, r) * wf(r);
+            }
+        }
+        return acc;
+    }
+}
+
+template <int ORDER, typename WF>
+__device__ __forceinline__ float gather_fwd_T(WF wf, int c, int n)
+{
+    if constexpr (ORDER <= 2) {
+        float acc = 0.0f;
+        if (c >= 1) acc += wf(-1);
+        acc -= wf(0);
+        return acc;
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float acc = 0.0f;
+        if (c >= 2 * R - 1 && c + 2 * R <= n) {
+#pragma unroll
+            for (int r = 1; r <= R; ++r) {
+                const float kc = stencil_c(R, r);
+                acc += kc * wf(-r);
+                acc -= kc * wf(r - 1);
+            }
+            return acc;
+        }
+        for (int r = 1; r <= R; ++r) {
+            const int sp = c - r;
+            if (sp >= 0) {
+                const int rs = fit_fwd(sp, n, R);
+                if (r <= rs) acc += stencil_c(rs, r) * wf(-r);
+            }
+            const int sm = c + (r - 1);
+            if (sm < n) {
+                const int rs = fit_fwd(sm, n, R);
+                if (r <= rs) acc -= stencil_c(rs, r) * wf(r - 1);
+            }
+        }
+        return acc;
+    }
+}
+
+// (E-CPML + E update)^T: lambda_H += D-^T w(lambda_E). lambda_E is left
+// unscaled here; the ca scaling happens in k_adj_h.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_adj_e(Grid g, const float* __restrict__ ccurl, CF3 lamE, F3 lamH,
+                                               Cpml wts, int has_cpml)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.plane) return;
+    const int i = blockIdx.y, s = blockIdx.z;
+    const int j = p / g.N[2];
+    const int k = p - j * g.N[2];
+    const int c[3] = {i, j, k};
+    const long long n = (long long)s * g.cells + (long long)i * g.st0 + p;
+    const bool cpml = has_cpml != 0;
+#pragma unroll
+    for (int u = 2; u >= 0; --u) {
+        float acc = lamH.c[u][n];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a == u) continue;
+            auto wf = [&](int off) {
+                int q[3] = {c[0], c[1], c[2]};
+                q[a] += off;
+                return channel_weight<true>(g, lamE, ccurl, wts, cpml, s, u, a, q);
+            };
+            acc += gather_bwd_T<ORDER>(wf, c[a], g.N[a]);
+        }
+        lamH.c[u][n] = acc;
+    }
+}
+
+// (H-CPML + H update)^T plus the deferred E scaling:
+// lambda_E = ca lambda_E (updated components) + D+^T w(lambda_H).
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_adj_h(Grid g, const float* __restrict__ ca, const float* __restrict__ hcurl,
+                                               CF3 lamH, F3 lamE, Cpml wts, int has_cpml)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.plane) return;
+    const int i = blockIdx.y, s = blockIdx.z;
+    const int j = p / g.N[2];
+    const int k = p - j * g.N[2];
+    const int c[3] = {i, j, k};
+    const long long m = (long long)i * g.st0 + p;
+    const long long n = (long long)s * g.cells + m;
+    const bool cpml = has_cpml != 0;
+    const float cav = ca[m];
+#pragma unroll
+    for (int t = 2; t >= 0; --t) {
+        float acc = lamE.c[t][n];
+        if (e_updated(g, t, c)) acc = cav * acc;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a == t) continue;
+            auto wf = [&](int off) {
+                int q[3] = {c[0], c[1], c[2]};
+                q[a] += off;
+                return channel_weight<false>(g, lamH, hcurl, wts, cpml, s, t, a, q);
+            };
+            acc += gather_fwd_T<ORDER>(wf, c[a], g.N[a]);
+        }
+        lamE.c[t][n] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+
+dim3 cell_grid(const Grid& g)
+{
+    unsigned bx = (unsigned)((g.plane + 255) / 256);
+    // keep the x-planes of one column of blocks on the same XCD (8 XCDs,
+    // round-robin block placement) so x-neighbours are served by its L2
+    if (bx >= 8) bx = (bx + 7) / 8 * 8;
+    return dim3(bx, (unsigned)g.N[0], (unsigned)g.nshot);
+}
+
+void launch_coefficients(const Grid& g, const float* eps, const float* sig, const float* mu, float* const co[6],
+                         float dt, hipStream_t st)
+{
+    unsigned blocks = (unsigned)((g.cells + 255) / 256);
+    hipLaunchKernelGGL(k_coefficients, dim3(blocks), dim3(256), 0, st, g, eps, sig, mu, co[0], co[1], co[2],
+                       co[3], co[4], co[5], dt);
+}
+
+template <int ORDER>
+static void fwd_step_impl(const Grid& g, const StepArgs& a, hipStream_t st)
+{
+    dim3 grid = cell_grid(g);
+    hipLaunchKernelGGL(k_fwd_h<ORDER>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml);
+    hipLaunchKernelGGL(k_fwd_e<ORDER>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
+                       a.has_cpml, a.src, a.hist);
+}
+
+void launch_forward_step(int order, const Grid& g, const StepArgs& a, hipStream_t st)
+{
+    if (order == 8) fwd_step_impl<8>(g, a, st);
+    else if (order == 4) fwd_step_impl<4>(g, a, st);
+    else fwd_step_impl<2>(g, a, st);
+}
+
+void launch_sample(const Grid& g, const float* f, const int* loc, float* out, int nrx, int nt, int it,
+                   hipStream_t st)
+{
+    int total = g.nshot * nrx;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_sample, dim3((total + 255) / 256), dim3(256), 0, st, g, f, loc, out, nrx, nt, it);
+}
+
+void launch_save_frame(const Grid& g, CF3 E, const HistWrite& hw, hipStream_t st)
+{
+    long long total = (long long)g.nshot * hw.box.cells;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_save_frame, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, g, E, hw);
+}
+
+void launch_quant_int8(const Grid& g, CF3 E, const float* src, long long src_cstride, const Int8Write& w,
+                       hipStream_t st)
+{
+    int comps = w.mode == 3 ? 3 : 1;
+    dim3 grid((unsigned)w.tiles, (unsigned)g.nshot, (unsigned)comps);
+    hipLaunchKernelGGL(k_quant_int8, grid, dim3(w.bx * w.by * w.bz), 0, st, g, E, src, src_cstride, w);
+}
+
+void launch_adj_receivers(const Grid& g, const ReceiverGroups& rg, float* lam, const float* dgrad, int nrx,
+                          int nt, int it, hipStream_t st)
+{
+    if (rg.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_adj_receivers, dim3((rg.ngroups + 255) / 256), dim3(256), 0, st, g, rg, lam, dgrad,
+                       nrx, nt, it);
+}
+
+void launch_adj_source(const Grid& g, const int* loc, const float* cb, const float* lam, float* gsrc, int nsrc,
+                       int nt, int it, float geo, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_source, dim3((nsrc + 63) / 64), dim3(64), 0, st, g, loc, cb, lam, gsrc, nsrc, nt,
+                       it, geo);
+}
+
+void launch_material_gradient(const Grid& g, CF3 L, const MatGrad& mg, void* geps, void* gsig, bool fp64,
+                              hipStream_t st)
+{
+    unsigned blocks = (unsigned)((mg.box.cells + 255) / 256);
+    if (fp64)
+        hipLaunchKernelGGL(k_material_gradient<double>, dim3(blocks), dim3(256), 0, st, g, L, mg, (double*)geps,
+                           (double*)gsig);
+    else
+        hipLaunchKernelGGL(k_material_gradient<float>, dim3(blocks), dim3(256), 0, st, g, L, mg, (float*)geps,
+                           (float*)gsig);
+}
+
+void launch_add_f64(const double* a, float* out, long long count, hipStream_t st)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_add_f64_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, a, out, count);
+}
+
+static long long max_face_cells(const Grid& g)
+{
+    long long best = 0;
+    for (int f = 0; f < 6; ++f) {
+        int axis = f >> 1;
+        int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
+        long long cells = (long long)g.pml[f] * g.N[a1] * g.N[a2];
+        if (cells > best) best = cells;
+    }
+    return best;
+}
+
+template <int ORDER>
+static void adj_step_impl(const Grid& g, const AdjArgs& a, hipStream_t st)
+{
+    dim3 grid = cell_grid(g);
+    long long fc = max_face_cells(g);
+    dim3 pgrid((unsigned)((fc + 255) / 256), 6, (unsigned)g.nshot);
+    if (a.has_cpml && fc > 0)
+        hipLaunchKernelGGL(k_adj_pml<true>, pgrid, dim3(256), 0, st, g, a.cb, a.lEc, a.cpe, a.we);
+    hipLaunchKernelGGL(k_adj_e<ORDER>, grid, dim3(256), 0, st, g, a.ccurl, a.lEc, a.lH, a.we, a.has_cpml);
+    if (a.has_cpml && fc > 0)
+        hipLaunchKernelGGL(k_adj_pml<false>, pgrid, dim3(256), 0, st, g, a.hrhs, a.lHc, a.cph, a.wh);
+    hipLaunchKernelGGL(k_adj_h<ORDER>, grid, dim3(256), 0, st, g, a.ca, a.hcurl, a.lHc, a.lE, a.wh, a.has_cpml);
+}
+
+void launch_adjoint_step(int order, const Grid& g, const AdjArgs& a, hipStream_t st)
+{
+    if (order == 8) adj_step_impl<8>(g, a, st);
+    else if (order == 4) adj_step_impl<4>(g, a, st);
+    else adj_step_impl<2>(g, a, st);
+}
+
+}  // namespace dgx
