@@ -1,0 +1,297 @@
+#!/usr/bin/env python3
+"""Benchmark of the FDTD hot path: forward + exact adjoint (material gradient)
+of a seeded synthetic workload on one MI355X, through the public ``compute``
+API and ``loss.backward()``.
+
+One step = one forward + adjoint solve of every shot of the workload. Prints
+one JSON line with the whole-job throughput in Gcell-updates/s (extended grid
+cells incl. CPML x time steps x shots x 2 directions), the roofline of the
+dominant kernel (HIP events around each of its launches, on the launch
+stream) and the reference CPU backend timed on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3"))
+    ap.add_argument("--nshot", type=int, default=None)
+    ap.add_argument("--nt", type=int, default=None)
+    ap.add_argument("--storage", default="float32", choices=("float32", "bfloat16", "float16", "int8"))
+    ap.add_argument("--rhs", default="auto", choices=("auto", "stored", "reconstructed"))
+    ap.add_argument("--accumulator", default="float64", choices=("float32", "float64"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--cpu-baseline-child", default=None, help=argparse.SUPPRESS)
+    return ap.parse_args()
+
+
+def _workload(args):
+    from deepgpr_amd import synthetic
+
+    kw = {}
+    if args.nshot is not None:
+        kw["nshot"] = args.nshot
+    if args.nt is not None:
+        kw["nt"] = args.nt
+    return synthetic.WORKLOADS[args.workload](**kw) if args.workload != "c1" else synthetic.c1_small_2d()
+
+
+def _compute_kwargs(w, args):
+    import torch
+
+    kw = dict(pmlthick=w.pml, fdtd_order=w.fdtd_order, mode=w.mode, model_gradient_sampling_interval=w.sampling_interval,
+              wavefield_rhs_history=args.rhs, gradient_accumulator=args.accumulator, **w.kwargs)
+    if args.storage == "int8":
+        kw["wavefield_compression"] = "int8"
+    else:
+        kw["wavefield_storage_dtype"] = {"float32": torch.float32, "bfloat16": torch.bfloat16,
+                                         "float16": torch.float16}[args.storage]
+    return kw
+
+
+# ----------------------------------------------------------------------------- byte models
+
+
+def _psi_elems(w):
+    """Active CPML auxiliary elements per shot (E and H), 2D TM or 3D."""
+    nx, ny = w.physical_shape[0] + 2 * w.pml, w.physical_shape[1] + 2 * w.pml
+    p = w.pml
+    if len(w.physical_shape) == 2:
+        # x faces: E phi2 (p+1, ny+1), H phi1 (p, ny+1); y faces: E phi2 (nx+1, p+1), H phi1 (nx+1, p)
+        e = 2 * ((p + 1) * (ny + 1) + (nx + 1) * (p + 1))
+        h = 2 * (p * (ny + 1) + (nx + 1) * p)
+        return e, h
+    nz = w.physical_shape[2] + 2 * p
+    e = h = 0
+    for a, b in ((ny, nz), (nx, nz), (nx, ny)):
+        e += 2 * 2 * (p + 1) * (a + 1) * (b + 1)
+        h += 2 * 2 * p * (a + 1) * (b + 1)
+    return e, h
+
+
+def kernel_bytes(kernel: str, w, args) -> float:
+    """Algorithmic HBM bytes of ONE launch of a kernel (one time step, all shots):
+    the compulsory loads/stores of that kernel's function, each array element
+    counted once per launch."""
+    shots = w.source_location.shape[0]
+    cells = w.extended_cells() * shots
+    two_d = len(w.physical_shape) == 2
+    comps = 3 if w.mode == 3 else 1
+    pe, ph = (x * shots for x in _psi_elems(w))
+    hist_b = {"float32": 4, "bfloat16": 2, "float16": 2, "int8": 1}[args.storage]
+    e_only = args.rhs == "reconstructed" or (args.rhs == "auto" and args.storage == "float32" and w.sampling_interval == 1)
+    if kernel == "fwd_h":
+        core = 24 if two_d else 40      # E read, H read+write, ch_curl
+        return cells * core + 8 * ph
+    if kernel == "fwd_e":
+        core = 24 if two_d else 48      # H read, E read+write, ca, ce_curl (+ce_rhs in 3D)
+        per_save = comps * hist_b * (1 if e_only else 2) / w.sampling_interval
+        return cells * (core + per_save) + 8 * pe
+    if kernel == "adj_e":
+        core = 24 if two_d else 40      # lambda_E read, ce_curl, lambda_H read+write
+        return cells * core + 4 * pe
+    if kernel == "adj_h":
+        core = 28 if two_d else 44      # lambda_H read, ch_curl, lambda_E read+write, ca
+        return cells * core + 4 * ph
+    if kernel == "material_gradient":
+        acc = 8 if args.accumulator == "float64" else 4
+        return cells * (comps * 4 + 2 * comps * hist_b + 12 + 2 * 2 * acc)
+    if kernel in ("adj_pml_e", "adj_pml_h"):
+        return 12 * (pe if kernel == "adj_pml_e" else ph)
+    return 0.0
+
+
+def step_bytes(w, args) -> float:
+    """Algorithmic bytes of one fwd+adj cell-update pair summed over the step's kernels."""
+    k = ("fwd_h", "fwd_e", "adj_e", "adj_h")
+    total = sum(kernel_bytes(n, w, args) for n in k)
+    total += kernel_bytes("material_gradient", w, args) / w.sampling_interval
+    return total
+
+
+# ----------------------------------------------------------------------------- CPU baseline
+
+
+def _cpu_baseline_child(spec: str) -> None:
+    """Runs in a subprocess: the reference CPU backend (or the oracle) on a
+    bounded sample of the workload; prints one JSON line."""
+    import numpy as np
+    import torch
+
+    from oracle import harness as H
+
+    args = json.loads(spec)
+    from deepgpr_amd import synthetic
+    from deepgpr_amd.preprocessing.model_setup import initialization, shift_locations
+    from deepgpr_amd.solver.pml import build_pml_coeffs
+
+    w = synthetic.WORKLOADS[args["workload"]]() if args["workload"] != "c1" else synthetic.c1_small_2d()
+    prep = initialization("cpu", w.eps_r, w.sigma, None, w.source_amplitudes, w.source_location[:1],
+                          w.receiver_location[:1], w.dx, w.dt, w.pml, w.fdtd_order)
+    coeffs = build_pml_coeffs(prep.er, prep.mr, w.dt, w.dx, prep.nx, prep.ny, prep.nz, prep.pmlthick, "cpu",
+                              torch.float32)[6:]
+    src = shift_locations(w.source_location[:1], prep.pmlthick, "cpu").numpy()
+    rx = shift_locations(w.receiver_location[:1], prep.pmlthick, "cpu").numpy()
+    NX, NY, NZ = prep.nx + 1, prep.ny + 1, prep.nz + 1
+    kind = "reference" if H.reference_available() else "port"
+    which = "ref" if kind == "reference" else "oracle"
+
+    def problem(nt):
+        p = H.make_problem(NX, NY, NZ, order=w.fdtd_order, nt=nt, nshot=1, nsrc=1, nrx=rx.shape[1], seed=1,
+                           random_state=False, pml=[int(v) for v in prep.pmlthick], d=(w.dx, w.dx, w.dx))
+        p.dt = w.dt
+        p.eps, p.sig, p.mu = (t.detach().numpy().astype(np.float32) for t in (prep.ere, prep.see, prep.mr))
+        p.coef_e = [coeffs[2 * f].numpy() for f in range(6)]
+        p.coef_h = [coeffs[2 * f + 1].numpy() for f in range(6)]
+        p.src_loc, p.rx_loc = src.astype(np.int32), rx.astype(np.int32)
+        p.wave = w.source_amplitudes[0, :nt, 0].numpy()[None].astype(np.float32)
+        p.mode, p.S = w.mode, w.sampling_interval
+        p.storage_type = H.RHS_FROM_E if w.sampling_interval == 1 else 0
+        p.data_grad = np.random.default_rng(0).standard_normal((1, nt, rx.shape[1])).astype(np.float32)
+        return p
+
+    calib = min(40, w.nt)
+    t0 = time.perf_counter()
+    H.run_cpu(problem(calib), which)
+    per_step = (time.perf_counter() - t0) / calib
+    nt = int(max(calib, min(w.nt, args["budget"] / max(per_step, 1e-9))))
+    p = problem(nt)
+    t0 = time.perf_counter()
+    H.run_cpu(p, which)
+    elapsed = time.perf_counter() - t0
+    cells = (NX - 1) * (NY - 1) * (NZ - 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    print(json.dumps({"value": 2 * cells * nt / elapsed / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+                      "kind": kind, "sample": f"{w.name}: shot 1 of {w.source_location.shape[0]}, nt={nt} of "
+                      f"{w.nt}, forward+adjoint, FP32 history, OpenMP threads={threads}, {elapsed:.2f} s"}))
+
+
+def _cpu_baseline(args):
+    spec = json.dumps({"workload": args.workload, "budget": args.cpu_budget_s})
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
+    env.setdefault("OMP_PROC_BIND", "close")
+    env.setdefault("OMP_PLACES", "cores")
+    res = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", spec],
+                         capture_output=True, text=True, env=env, timeout=600)
+    if res.returncode != 0:
+        return {"value": None, "unit": "Gcell-updates/s", "cores": None, "kind": None,
+                "sample": "failed: " + res.stderr.strip()[-300:]}
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+# ----------------------------------------------------------------------------- main
+
+
+def main():
+    args = _parse()
+    if args.cpu_baseline_child is not None:
+        _cpu_baseline_child(args.cpu_baseline_child)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported in this build "
+              "(shots shard across GPUs with one gradient all-reduce; not run here).", file=sys.stderr)
+        return 2
+    import torch
+
+    import deepgpr_amd as dg
+    from deepgpr_amd.native.loader import get_library
+
+    dev = torch.device("cuda:0")
+    w = _workload(args)
+    kw = _compute_kwargs(w, args)
+    eps = w.eps_r.to(dev)
+    sig = w.sigma.to(dev)
+    src_amp = w.source_amplitudes.to(dev)
+    src_loc = w.source_location.to(dev)
+    rx_loc = w.receiver_location.to(dev)
+
+    def step():
+        e = eps.detach().requires_grad_(True)
+        s = sig.detach().requires_grad_(True)
+        out = dg.compute(dev, w.dx, w.dt, src_amp, src_loc, rx_loc, eps_r=e, sigma=s, **kw)
+        data = out[-1]
+        (data * data).sum().backward()
+        return e.grad, s.grad
+
+    print(f"[bench] {w.name} storage={args.storage} rhs={args.rhs} acc={args.accumulator} "
+          f"warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step()
+        torch.cuda.synchronize(dev)
+        print(f"[bench] step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / max(args.steps, 1)
+    value = w.cell_updates() / (ms / 1e3) / 1e9
+
+    roofline = None
+    kernels = {}
+    if not args.no_kernel_timing:
+        lib = get_library()
+        lib.dgx_set_kernel_timing(1)
+        step()
+        torch.cuda.synchronize(dev)
+        lib.dgx_set_kernel_timing(0)
+        for kid in range(64):
+            name = lib.dgx_kernel_name(kid)
+            if not name:
+                break
+            tot, cnt = ctypes.c_double(), ctypes.c_longlong()
+            lib.dgx_kernel_timing(kid, ctypes.byref(tot), ctypes.byref(cnt))
+            if cnt.value:
+                kernels[name.decode()] = (tot.value, cnt.value)
+        lib.dgx_set_kernel_timing(0)
+        ranked = sorted(((t, n) for n, (t, c) in kernels.items() if kernel_bytes(n, w, args) > 0), reverse=True)
+        if ranked:
+            _, dom = ranked[0]
+            tot, cnt = kernels[dom]
+            avg_s = tot / cnt / 1e3
+            bytes_launch = kernel_bytes(dom, w, args)
+            achieved = bytes_launch / avg_s / 1e9
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_launch)}
+    pair_bytes = step_bytes(w, args) / (w.extended_cells() * w.source_location.shape[0])
+    cpu = None if args.no_cpu_baseline else _cpu_baseline(args)
+    line = {
+        "metric": "Gcell-updates/s fwd+adjoint (% HBM roofline); serve p50/p99 ms vs plain solve",
+        "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "shots": int(w.source_location.shape[0]), "nt": w.nt,
+                   "fdtd_order": w.fdtd_order, "history": args.storage, "rhs_history": args.rhs,
+                   "gradient_accumulator": args.accumulator, "sampling_interval": w.sampling_interval},
+        "roofline": roofline,
+        "step_roofline": {"algorithmic_bytes_per_cell_pair": round(pair_bytes, 1),
+                          "frac": round(value * 1e9 / 2 * pair_bytes / (HBM_PEAK_GBS * 1e9), 4)},
+        "kernels_ms_per_launch": {n: round(t / c, 4) for n, (t, c) in kernels.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

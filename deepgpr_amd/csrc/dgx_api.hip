@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "deepgpr_amd.h"
@@ -47,6 +48,74 @@ struct DevBuf {
             set_error("%s: %s", (what), hipGetErrorString(err_));                   \
             return -1;                                                              \
         }                                                                           \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Optional per-launch timing: HIP events recorded on the launch stream around
+// each launch of the hot kernels, summed per kernel after the call's final
+// synchronisation. Off by default; enabled by dgx_set_kernel_timing(1).
+// ---------------------------------------------------------------------------
+std::mutex g_timing_mu;
+int g_timing_on = 0;
+double g_timing_ms[kKCount];
+long long g_timing_count[kKCount];
+const char* const kKernelNames[kKCount] = {
+    "coefficients", "fwd_h", "fwd_e", "sample", "quant_int8", "save_frame", "adj_receivers",
+    "adj_source", "material_gradient", "adj_pml_e", "adj_e", "adj_pml_h", "adj_h"};
+
+struct LaunchTimer {
+    bool on;
+    hipStream_t stream;
+    struct Rec {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    explicit LaunchTimer(hipStream_t s) : stream(s)
+    {
+        std::lock_guard<std::mutex> lock(g_timing_mu);
+        on = g_timing_on != 0;
+    }
+    ~LaunchTimer()
+    {
+        for (Rec& r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+    int begin(int id)
+    {
+        if (!on) return -1;
+        Rec r{id, nullptr, nullptr};
+        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+        (void)hipEventRecord(r.a, stream);
+        recs.push_back(r);
+        return (int)recs.size() - 1;
+    }
+    void end(int h)
+    {
+        if (h >= 0) (void)hipEventRecord(recs[h].b, stream);
+    }
+    // call after the stream has been synchronised
+    void collect()
+    {
+        if (!on) return;
+        std::lock_guard<std::mutex> lock(g_timing_mu);
+        for (Rec& r : recs) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                g_timing_ms[r.id] += ms;
+                g_timing_count[r.id] += 1;
+            }
+        }
+    }
+};
+
+#define TIMED(timer, id, call)                 \
+    do {                                       \
+        int th_ = (timer).begin(id);           \
+        call;                                  \
+        (timer).end(th_);                      \
     } while (0)
 
 constexpr int kFlagRhsFromE = 1 << 4;
@@ -305,6 +374,73 @@ Int8View int8_view(const Plan& pl, const void* packed, int frames, int frame)
     return v;
 }
 
+constexpr int kFlagTm2d = 1 << 6;
+
+// The 2D TM fast path serves storage_type's SOLVER_TM2D requests on nz = 1
+// grids with an Ez source whose sources and receivers sit on the k = 0 layer
+// (the caller guarantees zero Ex/Ey/Hz states, deepgpr.h:77-83); everything
+// else runs the general path, which gives the same results.
+bool tm2d_applies(const dgx_solve* P, const Plan& pl, const std::vector<int>& sloc, const std::vector<int>& rloc)
+{
+    if (!(P->storage_type & kFlagTm2d) || pl.g.N[2] != 2 || P->source_component != 2 || pl.kind == kInt8) return false;
+    for (size_t q = 2; q < sloc.size(); q += 3)
+        if (sloc[q] != 0) return false;
+    for (size_t q = 2; q < rloc.size(); q += 3)
+        if (rloc[q] != 0) return false;
+    return true;
+}
+
+// Compact buffers of the 2D TM path: 3 fields (nshot * plane) and 5
+// coefficient planes.
+struct Tm2dState {
+    Tm2dGrid g;
+    Tm2dFields f;
+    DevBuf buf;
+    int setup(const Plan& pl, const dgx_solve* P, hipStream_t stream)
+    {
+        g.NX = pl.g.N[0];
+        g.NY = pl.g.N[1];
+        g.plane = g.NX * g.NY;
+        g.tiles = (g.plane + 255) / 256;
+        g.nshot = pl.g.nshot;
+        for (int a = 0; a < 3; ++a) g.d[a] = pl.g.d[a];
+        for (int f4 = 0; f4 < 4; ++f4) g.pml[f4] = pl.g.pml[f4];
+        g.has_cpml = pl.has_cpml;
+        const long long field = (long long)g.nshot * g.plane;
+        if (!buf.alloc(((size_t)3 * field + 5 * (size_t)g.plane) * sizeof(float))) return -1;
+        float* b = (float*)buf.p;
+        f.Ez = b;
+        f.Hx = b + field;
+        f.Hy = b + 2 * field;
+        float* co = b + 3 * field;
+        const float* full[5] = {P->update_coeff[0], P->update_coeff[1], P->update_coeff[2], P->update_coeff[4],
+                                P->update_coeff[5]};
+        for (int c = 0; c < 5; ++c) launch_tm_pack(full[c], co + (long long)c * g.plane, g.plane, false, stream);
+        f.ca = co;
+        f.ccurl = co + g.plane;
+        f.cb = co + 2 * (long long)g.plane;
+        f.hcurl = co + 3 * (long long)g.plane;
+        f.hrhs = co + 4 * (long long)g.plane;
+        for (int f4 = 0; f4 < 4; ++f4) {
+            f.coef_e[f4] = P->pml_coeff[f4];
+            f.coef_h[f4] = P->pml_coeff[6 + f4];
+            f.psi_e[f4] = P->phi[4 * f4 + 1];
+            f.psi_h[f4] = P->phi[4 * f4 + 2];
+        }
+        launch_tm_pack(P->E[2], f.Ez, field, false, stream);
+        launch_tm_pack(P->H[0], f.Hx, field, false, stream);
+        launch_tm_pack(P->H[1], f.Hy, field, false, stream);
+        return 0;
+    }
+    void unpack(const dgx_solve* P, hipStream_t stream)
+    {
+        const long long field = (long long)g.nshot * g.plane;
+        launch_tm_pack(P->E[2], f.Ez, field, true, stream);
+        launch_tm_pack(P->H[0], f.Hx, field, true, stream);
+        launch_tm_pack(P->H[1], f.Hy, field, true, stream);
+    }
+};
+
 int run_forward(const dgx_solve* P, hipStream_t stream)
 {
     Plan pl;
@@ -351,7 +487,9 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         }
 
     const bool inject = P->nsource > 0 && P->source_component >= 0 && P->source_component <= 2;
-    long long nblocks = (long long)grid.x * grid.y * grid.z;
+    const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
+    const int tm_tiles = (g.N[0] * g.N[1] + 255) / 256;
+    long long nblocks = tm ? (long long)tm_tiles * g.nshot : (long long)grid.x * grid.y * grid.z;
     std::vector<int> boff((size_t)nblocks + 1, 0);
     std::vector<int2> ents;
     if (inject) {
@@ -368,6 +506,10 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
                 long long m = (long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2];
                 int p = L[1] * g.N[2] + L[2];
                 long long blk = (long long)(p / 256) + (long long)grid.x * (L[0] + (long long)grid.y * s);
+                if (tm) {  // compact plane index and 2D tile
+                    m = (long long)L[0] * g.N[1] + L[1];
+                    blk = (long long)(m / 256) + (long long)tm_tiles * s;
+                }
                 tmp.push_back({blk, s * P->nsource + k, (int)m, k});
             }
         std::stable_sort(tmp.begin(), tmp.end(), [](const E4& a, const E4& b) { return a.blk < b.blk; });
@@ -413,7 +555,8 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         }
     }
 
-    launch_coefficients(g, P->eps_r_pad, P->sigma_pad, P->mu_r_pad, co, P->dt, stream);
+    LaunchTimer tmr(stream);
+    TIMED(tmr, kKCoeff, launch_coefficients(g, P->eps_r_pad, P->sigma_pad, P->mu_r_pad, co, P->dt, stream));
 
     StepArgs a{};
     a.ca = co[0];
@@ -433,6 +576,11 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     a.src = src;
     const size_t eb = elem_bytes(pl.kind);
     const int rcomp = (P->receiver_component >= 0 && P->receiver_component <= 2) ? P->receiver_component : 2;
+    Tm2dState tms;
+    if (tm && tms.setup(pl, P, stream)) {
+        set_error("forward: out of device memory for the 2D TM state");
+        return -1;
+    }
 
     for (int it = 0; it < nt; ++it) {
         const bool save = pl.history && (it % S) == 0;
@@ -442,7 +590,8 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         hw.box = pl.box;
         if (save) {
             if (int8) {
-                launch_quant_int8(g, a.Ec, nullptr, 0, int8_writer(pl, P->E_saved, pl.nt_saved, ts), stream);
+                TIMED(tmr, kKQuant,
+                      launch_quant_int8(g, a.Ec, nullptr, 0, int8_writer(pl, P->E_saved, pl.nt_saved, ts), stream));
             } else {
                 hw.save_e = 1;
                 hw.e_kind = pl.kind;
@@ -465,13 +614,29 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         }
         a.hist = hw;
         a.src.it = it;
-        launch_forward_step(pl.order, g, a, stream);
+        if (tm) {
+            TIMED(tmr, kKFwdH, launch_tm_fwd_h(pl.order, tms.g, tms.f, stream));
+            TIMED(tmr, kKFwdE, launch_tm_fwd_e(pl.order, tms.g, tms.f, a.src, a.hist, stream));
+        } else {
+            TIMED(tmr, kKFwdH, launch_fwd_h(pl.order, g, a, stream));
+            TIMED(tmr, kKFwdE, launch_fwd_e(pl.order, g, a, stream));
+        }
         if (save && pl.store_r && int8)
-            launch_quant_int8(g, a.Ec, (const float*)d_rscratch.p, pl.snap,
-                              int8_writer(pl, P->R_saved, pl.nt_saved, ts), stream);
-        if (P->nreceiver > 0)
-            launch_sample(g, P->E[rcomp], P->receiver_location, P->receiver_data, P->nreceiver, nt, it, stream);
+            TIMED(tmr, kKQuant,
+                  launch_quant_int8(g, a.Ec, (const float*)d_rscratch.p, pl.snap,
+                                    int8_writer(pl, P->R_saved, pl.nt_saved, ts), stream));
+        if (P->nreceiver > 0) {
+            if (tm && rcomp == 2)
+                TIMED(tmr, kKSample,
+                      launch_sample(g, tms.f.Ez, P->receiver_location, P->receiver_data, P->nreceiver, nt, it,
+                                    stream, tms.g.plane, tms.g.NY, 1));
+            else
+                TIMED(tmr, kKSample,
+                      launch_sample(g, P->E[rcomp], P->receiver_location, P->receiver_data, P->nreceiver, nt, it,
+                                    stream));
+        }
     }
+    if (tm) tms.unpack(P, stream);
     if (pl.e_only) {
         if (int8) {
             launch_quant_int8(g, a.Ec, nullptr, 0, int8_writer(pl, P->R_saved, 1, 0), stream);
@@ -482,11 +647,12 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
             hw.e_kind = pl.kind;
             hw.e_base = P->R_saved;
             hw.e_cstride = pl.snap;
-            launch_save_frame(g, a.Ec, hw, stream);
+            TIMED(tmr, kKSaveFrame, launch_save_frame(g, a.Ec, hw, stream));
         }
     }
     DGX_TRY(hipGetLastError(), "forward: kernel launch");
     DGX_TRY(hipStreamSynchronize(stream), "forward: stream synchronize");
+    tmr.collect();
     return 0;
 }
 
@@ -514,11 +680,11 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
                                hipMemcpyDeviceToHost, stream),
                 "backward: copy receiver locations");
     }
-    if (want_src) {
-        if (!P->source_location || !P->grad_source) {
-            set_error("backward: null source arrays");
-            return -1;
-        }
+    if (want_src && (!P->source_location || !P->grad_source)) {
+        set_error("backward: null source arrays");
+        return -1;
+    }
+    if (P->nsource > 0 && P->source_location) {
         hsloc.resize((size_t)g.nshot * P->nsource * 3);
         DGX_TRY(hipMemcpyAsync(hsloc.data(), P->source_location, hsloc.size() * sizeof(int),
                                hipMemcpyDeviceToHost, stream),
@@ -545,8 +711,18 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
         }
     }
 
-    // receiver groups: one per (shot, cell), members in receiver order
-    const bool adj_rx = P->nreceiver > 0 && P->receiver_component >= 0 && P->receiver_component <= 2;
+    const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
+    Grid gc = g;  // compact 2D TM addressing for the receiver/source kernels
+    if (tm) {
+        gc.st0 = g.N[1];
+        gc.N[2] = 1;
+        gc.cells = (long long)g.N[0] * g.N[1];
+    }
+
+    // receiver groups: one per (shot, cell), members in receiver order; on the
+    // 2D TM path only Ez receivers couple into the active fields
+    const bool adj_rx = P->nreceiver > 0 && P->receiver_component >= 0 && P->receiver_component <= 2 &&
+                        (!tm || P->receiver_component == 2);
     std::vector<int> goff(1, 0), gshot, gmem;
     std::vector<long long> gcell;
     if (adj_rx) {
@@ -554,7 +730,7 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             std::vector<std::pair<long long, int>> cells;
             for (int r = 0; r < P->nreceiver; ++r) {
                 const int* L = &hrloc[((size_t)s * P->nreceiver + r) * 3];
-                cells.push_back({(long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2], r});
+                cells.push_back({(long long)L[0] * gc.st0 + (long long)L[1] * gc.N[2] + L[2], r});
             }
             std::stable_sort(cells.begin(), cells.end(),
                              [](const std::pair<long long, int>& x, const std::pair<long long, int>& y) {
@@ -593,7 +769,26 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     // CPML weight scratch, shaped like the psi arrays
     DevBuf d_w;
     Cpml we{}, wh{};
-    if (pl.has_cpml) {
+    Tm2dWeights tw{};
+    if (pl.has_cpml && tm) {
+        long long tot = 0;
+        for (int f = 0; f < 4; ++f) {
+            long long across = f < 2 ? g.N[1] : g.N[0];
+            tot += (long long)g.nshot * across * (2 * g.pml[f] + 1);
+        }
+        if (!d_w.alloc((size_t)tot * sizeof(float))) {
+            set_error("backward: out of device memory for CPML weights");
+            return -1;
+        }
+        float* p = (float*)d_w.p;
+        for (int f = 0; f < 4; ++f) {
+            long long across = (long long)g.nshot * (f < 2 ? g.N[1] : g.N[0]);
+            tw.we[f] = p;
+            p += across * (g.pml[f] + 1);
+            tw.wh[f] = p;
+            p += across * g.pml[f];
+        }
+    } else if (pl.has_cpml) {
         long long tot = 0;
         for (int f = 0; f < 6; ++f)
             for (int w = 0; w < 2; ++w) tot += psi_elems(g, true, f, w) + psi_elems(g, false, f, w);
@@ -630,7 +825,8 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
         gsig = (double*)d_acc.p + ngrad;
     }
 
-    launch_coefficients(g, P->eps_r_pad, P->sigma_pad, P->mu_r_pad, co, P->dt, stream);
+    LaunchTimer tmr(stream);
+    TIMED(tmr, kKCoeff, launch_coefficients(g, P->eps_r_pad, P->sigma_pad, P->mu_r_pad, co, P->dt, stream));
 
     AdjArgs a{};
     a.ca = co[0];
@@ -663,13 +859,22 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     mg.box = pl.box;
     const size_t eb = elem_bytes(pl.kind);
     const float geo = want_src ? g.d[P->source_component] / (g.d[0] * g.d[1] * g.d[2]) : 0.0f;
+    Tm2dState tms;
+    if (tm && tms.setup(pl, P, stream)) {
+        set_error("backward: out of device memory for the 2D TM state");
+        return -1;
+    }
+    float* lam_rx = tm ? tms.f.Ez : (adj_rx ? P->E[P->receiver_component] : nullptr);
+    float* lam_src = tm ? tms.f.Ez : (want_src ? P->E[P->source_component] : nullptr);
+    const float* cb_src = tm ? tms.f.cb : co[2];
 
     for (int it = nt - 1; it >= 0; --it) {
         if (adj_rx)
-            launch_adj_receivers(g, rg, P->E[P->receiver_component], P->data_grad, P->nreceiver, nt, it, stream);
+            TIMED(tmr, kKAdjRx, launch_adj_receivers(gc, rg, lam_rx, P->data_grad, P->nreceiver, nt, it, stream));
         if (want_src)
-            launch_adj_source(g, P->source_location, co[2], P->E[P->source_component], P->grad_source, P->nsource,
-                              nt, it, geo, stream);
+            TIMED(tmr, kKAdjSrc,
+                  launch_adj_source(gc, P->source_location, cb_src, lam_src, P->grad_source, P->nsource, nt, it,
+                                    geo, stream));
         if (want_mat && it % S == 0) {
             int wgt = S;
             const int ts = it / S;
@@ -693,16 +898,32 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
                     mg.s_cstride = pl.snap;
                 }
             }
-            launch_material_gradient(g, a.lEc, mg, geps, gsig, fp64, stream);
+            if (tm)
+                TIMED(tmr, kKMatGrad,
+                      launch_tm_material_gradient(tms.g, tms.f, mg, pl.comps == 3 ? 2 : 0, geps, gsig, fp64, stream));
+            else
+                TIMED(tmr, kKMatGrad, launch_material_gradient(g, a.lEc, mg, geps, gsig, fp64, stream));
         }
-        launch_adjoint_step(pl.order, g, a, stream);
+        if (tm) {
+            if (a.has_cpml) TIMED(tmr, kKAdjPmlE, launch_tm_adj_pml(true, tms.g, tms.f, tw, stream));
+            TIMED(tmr, kKAdjE, launch_tm_adj_e(pl.order, tms.g, tms.f, tw, stream));
+            if (a.has_cpml) TIMED(tmr, kKAdjPmlH, launch_tm_adj_pml(false, tms.g, tms.f, tw, stream));
+            TIMED(tmr, kKAdjH, launch_tm_adj_h(pl.order, tms.g, tms.f, tw, stream));
+        } else {
+            if (a.has_cpml) TIMED(tmr, kKAdjPmlE, launch_adj_pml(true, g, a, stream));
+            TIMED(tmr, kKAdjE, launch_adj_e(pl.order, g, a, stream));
+            if (a.has_cpml) TIMED(tmr, kKAdjPmlH, launch_adj_pml(false, g, a, stream));
+            TIMED(tmr, kKAdjH, launch_adj_h(pl.order, g, a, stream));
+        }
     }
+    if (tm) tms.unpack(P, stream);
     if (want_mat && fp64) {
         if (P->eps_r_requires_grad == 1) launch_add_f64((const double*)geps, P->grad_eps_r, ngrad, stream);
         if (P->sigma_requires_grad == 1) launch_add_f64((const double*)gsig, P->grad_sigma, ngrad, stream);
     }
     DGX_TRY(hipGetLastError(), "backward: kernel launch");
     DGX_TRY(hipStreamSynchronize(stream), "backward: stream synchronize");
+    tmr.collect();
     return 0;
 }
 
@@ -795,6 +1016,31 @@ DGX_API long long dgx_scratch_bytes(const dgx_solve* solve, int backward)
         if (solve->accumulate_fp64) bytes += 16LL * (g.N[0] - 1) * (g.N[1] - 1) * (g.N[2] - 1);
     }
     return bytes;
+}
+
+DGX_API void dgx_set_kernel_timing(int enable)
+{
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    g_timing_on = enable ? 1 : 0;
+    if (!enable) return;  // totals stay readable after recording stops
+    for (int k = 0; k < kKCount; ++k) {
+        g_timing_ms[k] = 0.0;
+        g_timing_count[k] = 0;
+    }
+}
+
+DGX_API int dgx_kernel_timing(int kernel, double* total_ms, long long* launches)
+{
+    if (kernel < 0 || kernel >= kKCount) return -1;
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    if (total_ms) *total_ms = g_timing_ms[kernel];
+    if (launches) *launches = g_timing_count[kernel];
+    return 0;
+}
+
+DGX_API const char* dgx_kernel_name(int kernel)
+{
+    return (kernel >= 0 && kernel < kKCount) ? kKernelNames[kernel] : "";
 }
 
 // --- reference ABI-6 export set ----------------------------------------------

@@ -124,6 +124,78 @@ __device__ __forceinline__ float dfwd(const float* __restrict__ f, long long id,
     }
 }
 
+// Transposes of the staggered differences (deepgpr_cpu.c:430-511).
+template <int ORDER, typename WF>
+__device__ __forceinline__ float gather_bwd_T(WF wf, int c, int n)
+{
+    if constexpr (ORDER <= 2) {
+        float acc = wf(0);
+        if (c + 1 < n) acc -= wf(1);
+        return acc;
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float acc = 0.0f;
+        if (c >= 2 * R - 1 && c + 2 * R - 1 < n) {
+#pragma unroll
+            for (int r = 1; r <= R; ++r) {
+                const float kc = stencil_c(R, r);
+                acc += kc * wf(-(r - 1));
+                acc -= kc * wf(r);
+            }
+            return acc;
+        }
+        for (int r = 1; r <= R; ++r) {
+            const int sp = c - (r - 1);
+            if (sp >= 0) {
+                const int rs = fit_bwd(sp, n, R);
+                if (r <= rs) acc += stencil_c(rs, r) * wf(-(r - 1));
+            }
+            const int sm = c + r;
+            if (sm < n) {
+                const int rs = fit_bwd(sm, n, R);
+                if (r <= rs) acc -= stencil_c(rs, r) * wf(r);
+            }
+        }
+        return acc;
+    }
+}
+
+template <int ORDER, typename WF>
+__device__ __forceinline__ float gather_fwd_T(WF wf, int c, int n)
+{
+    if constexpr (ORDER <= 2) {
+        float acc = 0.0f;
+        if (c >= 1) acc += wf(-1);
+        acc -= wf(0);
+        return acc;
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float acc = 0.0f;
+        if (c >= 2 * R - 1 && c + 2 * R <= n) {
+#pragma unroll
+            for (int r = 1; r <= R; ++r) {
+                const float kc = stencil_c(R, r);
+                acc += kc * wf(-r);
+                acc -= kc * wf(r - 1);
+            }
+            return acc;
+        }
+        for (int r = 1; r <= R; ++r) {
+            const int sp = c - r;
+            if (sp >= 0) {
+                const int rs = fit_fwd(sp, n, R);
+                if (r <= rs) acc += stencil_c(rs, r) * wf(-r);
+            }
+            const int sm = c + (r - 1);
+            if (sm < n) {
+                const int rs = fit_fwd(sm, n, R);
+                if (r <= rs) acc -= stencil_c(rs, r) * wf(r - 1);
+            }
+        }
+        return acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Update masks (deepgpr_cpu.c:697-699, 913-915)
 // ---------------------------------------------------------------------------

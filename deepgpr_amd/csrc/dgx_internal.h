@@ -87,12 +87,50 @@ struct AdjArgs {
     int has_cpml;
 };
 
+// 2D Ez-TM fast path: compact (nshot, NX, NY) state, y fastest.
+struct Tm2dGrid {
+    int NX, NY;
+    int plane;   // NX * NY
+    int tiles;   // ceil(plane / 256)
+    int nshot;
+    float d[3];
+    int pml[4];  // x0, xm, y0, ym
+    int has_cpml;
+};
+
+struct Tm2dFields {
+    float *Ez, *Hx, *Hy;                           // or their adjoints
+    const float *ca, *ccurl, *cb, *hcurl, *hrhs;   // compact k = 0 planes
+    const float* coef_e[4];
+    const float* coef_h[4];
+    float* psi_e[4];  // E phi2 of x0, xm, y0, ym (Ez couplings)
+    float* psi_h[4];  // H phi1 of x0, xm, y0, ym (Hy, Hx couplings)
+};
+
+struct Tm2dWeights {
+    float* we[4];
+    float* wh[4];
+};
+
+void launch_tm_fwd_h(int order, const Tm2dGrid& g, const Tm2dFields& f, hipStream_t st);
+void launch_tm_fwd_e(int order, const Tm2dGrid& g, const Tm2dFields& f, const SourceBlocks& src,
+                     const HistWrite& hw, hipStream_t st);
+void launch_tm_pack(const float* full, float* compact, long long count, bool unpack, hipStream_t st);
+void launch_tm_adj_pml(bool electric, const Tm2dGrid& g, const Tm2dFields& f, const Tm2dWeights& w,
+                       hipStream_t st);
+void launch_tm_adj_e(int order, const Tm2dGrid& g, const Tm2dFields& f, const Tm2dWeights& w, hipStream_t st);
+void launch_tm_adj_h(int order, const Tm2dGrid& g, const Tm2dFields& f, const Tm2dWeights& w, hipStream_t st);
+void launch_tm_material_gradient(const Tm2dGrid& g, const Tm2dFields& f, const MatGrad& mg, int ez_comp,
+                                 void* geps, void* gsig, bool fp64, hipStream_t st);
+
 dim3 cell_grid(const Grid& g);
 void launch_coefficients(const Grid& g, const float* eps, const float* sig, const float* mu, float* const co[6],
                          float dt, hipStream_t st);
-void launch_forward_step(int order, const Grid& g, const StepArgs& a, hipStream_t st);
+void launch_fwd_h(int order, const Grid& g, const StepArgs& a, hipStream_t st);
+void launch_fwd_e(int order, const Grid& g, const StepArgs& a, hipStream_t st);
+// Receiver gather from a field with strides (shot, x, y); z stride is 1.
 void launch_sample(const Grid& g, const float* f, const int* loc, float* out, int nrx, int nt, int it,
-                   hipStream_t st);
+                   hipStream_t st, long long shot_stride = -1, long long x_stride = -1, long long y_stride = -1);
 void launch_save_frame(const Grid& g, CF3 E, const HistWrite& hw, hipStream_t st);
 void launch_quant_int8(const Grid& g, CF3 E, const float* src, long long src_cstride, const Int8Write& w,
                        hipStream_t st);
@@ -103,6 +141,14 @@ void launch_adj_source(const Grid& g, const int* loc, const float* cb, const flo
 void launch_material_gradient(const Grid& g, CF3 L, const MatGrad& mg, void* geps, void* gsig, bool fp64,
                               hipStream_t st);
 void launch_add_f64(const double* a, float* out, long long count, hipStream_t st);
-void launch_adjoint_step(int order, const Grid& g, const AdjArgs& a, hipStream_t st);
+void launch_adj_pml(bool electric, const Grid& g, const AdjArgs& a, hipStream_t st);
+void launch_adj_e(int order, const Grid& g, const AdjArgs& a, hipStream_t st);
+void launch_adj_h(int order, const Grid& g, const AdjArgs& a, hipStream_t st);
+
+// Kernel ids of the optional per-launch timing (dgx_set_kernel_timing).
+enum KernelId : int {
+    kKCoeff = 0, kKFwdH, kKFwdE, kKSample, kKQuant, kKSaveFrame, kKAdjRx, kKAdjSrc, kKMatGrad, kKAdjPmlE,
+    kKAdjE, kKAdjPmlH, kKAdjH, kKCount
+};
 
 }  // namespace dgx

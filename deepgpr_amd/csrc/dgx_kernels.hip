@@ -226,15 +226,16 @@ __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__
 }
 
 // Receiver gather: rx[s, it, r] = E_comp at the receiver (deepgpr_cpu.c:600-623)
-__global__ __launch_bounds__(256) void k_sample(Grid g, const float* __restrict__ f, const int* __restrict__ loc,
-                                                float* __restrict__ out, int nrx, int nt, int it)
+__global__ __launch_bounds__(256) void k_sample(int nshot, const float* __restrict__ f, const int* __restrict__ loc,
+                                                float* __restrict__ out, int nrx, int nt, int it, long long sst,
+                                                long long xst, long long yst)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.nshot * nrx) return;
+    if (idx >= nshot * nrx) return;
     int s = idx / nrx, r = idx - s * nrx;
     const int* L = loc + (long long)idx * 3;
-    long long m = (long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2];
-    out[((long long)s * nt + it) * nrx + r] = f[(long long)s * g.cells + m];
+    long long m = (long long)L[0] * xst + (long long)L[1] * yst + L[2];
+    out[((long long)s * nt + it) * nrx + r] = f[(long long)s * sst + m];
 }
 
 // Copy E components of the history box into one frame (final E^nt frame).
@@ -416,23 +417,27 @@ __global__ __launch_bounds__(256) void k_adj_pml(Grid g, const float* __restrict
     const int pp = g.pml[f];
     if (pp <= 0) return;
     const int axis = f >> 1;
-    const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
+    // the two transverse axes in memory order (slow, fast) so that
+    // consecutive threads touch consecutive addresses
+    const int slow = axis == 0 ? 1 : 0, fast = axis == 2 ? 1 : 2;
     const int n_face = pp;  // cells of the face region along its axis
-    const long long plane = (long long)g.N[a1] * g.N[a2];
+    const long long plane = (long long)g.N[slow] * g.N[fast];
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= plane * n_face) return;
     const int rf_idx = (int)(idx / plane);
     const long long rem = idx - (long long)rf_idx * plane;
-    const int ca1 = (int)(rem / g.N[a2]);
-    const int ca2 = (int)(rem - (long long)ca1 * g.N[a2]);
+    const int cs = (int)(rem / g.N[fast]);
+    const int cf = (int)(rem - (long long)cs * g.N[fast]);
     const int n_ax = g.N[axis];
     int cax;
     if ((f & 1) == 0) cax = ELECTRIC ? 1 + rf_idx : rf_idx;
     else cax = n_ax - 1 - pp + rf_idx;
+    if (cax < 0 || cax >= n_ax) return;
     int c[3];
     c[axis] = cax;
-    c[a1] = ca1;
-    c[a2] = ca2;
+    c[slow] = cs;
+    c[fast] = cf;
+    const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
     const int q = face_q(g, ELECTRIC, f, cax);
     if (q < 0) return;
     const long long m = (long long)c[0] * g.st0 + (long long)c[1] * g.N[2] + c[2];
@@ -490,78 +495,6 @@ __device__ __forceinline__ float channel_weight(const Grid& g, const CF3& lam, c
         acc += (plus ? k : -k) * lt;
     }
     return acc;
-}
-
-// Transposes of the staggered differences (deepgpr_cpu.c:430-511).
-template <int ORDER, typename WF>
-__device__ __forceinline__ float gather_bwd_T(WF wf, int c, int n)
-{
-    if constexpr (ORDER <= 2) {
-        float acc = wf(0);
-        if (c + 1 < n) acc -= wf(1);
-        return acc;
-    } else {
-        constexpr int R = Radius<ORDER>::value;
-        float acc = 0.0f;
-        if (c >= 2 * R - 1 && c + 2 * R - 1 < n) {
-#pragma unroll
-            for (int r = 1; r <= R; ++r) {
-                const float kc = stencil_c(R, r);
-                acc += kc * wf(-(r - 1));
-                acc -= kc * wf(r);
-            }
-            return acc;
-        }
-        for (int r = 1; r <= R; ++r) {
-            const int sp = c - (r - 1);
-            if (sp >= 0) {
-                const int rs = fit_bwd(sp, n, R);
-                if (r <= rs) acc += stencil_c(rs, r) * wf(-(r - 1));
-            }
-            const int sm = c + r;
-            if (sm < n) {
-                const int rs = fit_bwd(sm, n, R);
-                if (r <= rs) acc -= stencil_c(rs, r) * wf(r);
-            }
-        }
-        return acc;
-    }
-}
-
-template <int ORDER, typename WF>
-__device__ __forceinline__ float gather_fwd_T(WF wf, int c, int n)
-{
-    if constexpr (ORDER <= 2) {
-        float acc = 0.0f;
-        if (c >= 1) acc += wf(-1);
-        acc -= wf(0);
-        return acc;
-    } else {
-        constexpr int R = Radius<ORDER>::value;
-        float acc = 0.0f;
-        if (c >= 2 * R - 1 && c + 2 * R <= n) {
-#pragma unroll
-            for (int r = 1; r <= R; ++r) {
-                const float kc = stencil_c(R, r);
-                acc += kc * wf(-r);
-                acc -= kc * wf(r - 1);
-            }
-            return acc;
-        }
-        for (int r = 1; r <= R; ++r) {
-            const int sp = c - r;
-            if (sp >= 0) {
-                const int rs = fit_fwd(sp, n, R);
-                if (r <= rs) acc += stencil_c(rs, r) * wf(-r);
-            }
-            const int sm = c + (r - 1);
-            if (sm < n) {
-                const int rs = fit_fwd(sm, n, R);
-                if (r <= rs) acc -= stencil_c(rs, r) * wf(r - 1);
-            }
-        }
-        return acc;
-    }
 }
 
 // (E-CPML + E update)^T: lambda_H += D-^T w(lambda_E). lambda_E is left
@@ -650,28 +583,43 @@ void launch_coefficients(const Grid& g, const float* eps, const float* sig, cons
                        co[3], co[4], co[5], dt);
 }
 
-template <int ORDER>
-static void fwd_step_impl(const Grid& g, const StepArgs& a, hipStream_t st)
+void launch_fwd_h(int order, const Grid& g, const StepArgs& a, hipStream_t st)
 {
     dim3 grid = cell_grid(g);
-    hipLaunchKernelGGL(k_fwd_h<ORDER>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml);
-    hipLaunchKernelGGL(k_fwd_e<ORDER>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
-                       a.has_cpml, a.src, a.hist);
+    if (order == 8)
+        hipLaunchKernelGGL(k_fwd_h<8>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml);
+    else if (order == 4)
+        hipLaunchKernelGGL(k_fwd_h<4>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml);
+    else
+        hipLaunchKernelGGL(k_fwd_h<2>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml);
 }
 
-void launch_forward_step(int order, const Grid& g, const StepArgs& a, hipStream_t st)
+void launch_fwd_e(int order, const Grid& g, const StepArgs& a, hipStream_t st)
 {
-    if (order == 8) fwd_step_impl<8>(g, a, st);
-    else if (order == 4) fwd_step_impl<4>(g, a, st);
-    else fwd_step_impl<2>(g, a, st);
+    dim3 grid = cell_grid(g);
+    if (order == 8)
+        hipLaunchKernelGGL(k_fwd_e<8>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
+                           a.has_cpml, a.src, a.hist);
+    else if (order == 4)
+        hipLaunchKernelGGL(k_fwd_e<4>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
+                           a.has_cpml, a.src, a.hist);
+    else
+        hipLaunchKernelGGL(k_fwd_e<2>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
+                           a.has_cpml, a.src, a.hist);
 }
 
 void launch_sample(const Grid& g, const float* f, const int* loc, float* out, int nrx, int nt, int it,
-                   hipStream_t st)
+                   hipStream_t st, long long shot_stride, long long x_stride, long long y_stride)
 {
     int total = g.nshot * nrx;
     if (total <= 0) return;
-    hipLaunchKernelGGL(k_sample, dim3((total + 255) / 256), dim3(256), 0, st, g, f, loc, out, nrx, nt, it);
+    if (shot_stride < 0) {
+        shot_stride = g.cells;
+        x_stride = g.st0;
+        y_stride = g.N[2];
+    }
+    hipLaunchKernelGGL(k_sample, dim3((total + 255) / 256), dim3(256), 0, st, g.nshot, f, loc, out, nrx, nt, it,
+                       shot_stride, x_stride, y_stride);
 }
 
 void launch_save_frame(const Grid& g, CF3 E, const HistWrite& hw, hipStream_t st)
@@ -734,25 +682,37 @@ static long long max_face_cells(const Grid& g)
     return best;
 }
 
-template <int ORDER>
-static void adj_step_impl(const Grid& g, const AdjArgs& a, hipStream_t st)
+void launch_adj_pml(bool electric, const Grid& g, const AdjArgs& a, hipStream_t st)
 {
-    dim3 grid = cell_grid(g);
     long long fc = max_face_cells(g);
+    if (!a.has_cpml || fc <= 0) return;
     dim3 pgrid((unsigned)((fc + 255) / 256), 6, (unsigned)g.nshot);
-    if (a.has_cpml && fc > 0)
+    if (electric)
         hipLaunchKernelGGL(k_adj_pml<true>, pgrid, dim3(256), 0, st, g, a.cb, a.lEc, a.cpe, a.we);
-    hipLaunchKernelGGL(k_adj_e<ORDER>, grid, dim3(256), 0, st, g, a.ccurl, a.lEc, a.lH, a.we, a.has_cpml);
-    if (a.has_cpml && fc > 0)
+    else
         hipLaunchKernelGGL(k_adj_pml<false>, pgrid, dim3(256), 0, st, g, a.hrhs, a.lHc, a.cph, a.wh);
-    hipLaunchKernelGGL(k_adj_h<ORDER>, grid, dim3(256), 0, st, g, a.ca, a.hcurl, a.lHc, a.lE, a.wh, a.has_cpml);
 }
 
-void launch_adjoint_step(int order, const Grid& g, const AdjArgs& a, hipStream_t st)
+void launch_adj_e(int order, const Grid& g, const AdjArgs& a, hipStream_t st)
 {
-    if (order == 8) adj_step_impl<8>(g, a, st);
-    else if (order == 4) adj_step_impl<4>(g, a, st);
-    else adj_step_impl<2>(g, a, st);
+    dim3 grid = cell_grid(g);
+    if (order == 8)
+        hipLaunchKernelGGL(k_adj_e<8>, grid, dim3(256), 0, st, g, a.ccurl, a.lEc, a.lH, a.we, a.has_cpml);
+    else if (order == 4)
+        hipLaunchKernelGGL(k_adj_e<4>, grid, dim3(256), 0, st, g, a.ccurl, a.lEc, a.lH, a.we, a.has_cpml);
+    else
+        hipLaunchKernelGGL(k_adj_e<2>, grid, dim3(256), 0, st, g, a.ccurl, a.lEc, a.lH, a.we, a.has_cpml);
+}
+
+void launch_adj_h(int order, const Grid& g, const AdjArgs& a, hipStream_t st)
+{
+    dim3 grid = cell_grid(g);
+    if (order == 8)
+        hipLaunchKernelGGL(k_adj_h<8>, grid, dim3(256), 0, st, g, a.ca, a.hcurl, a.lHc, a.lE, a.wh, a.has_cpml);
+    else if (order == 4)
+        hipLaunchKernelGGL(k_adj_h<4>, grid, dim3(256), 0, st, g, a.ca, a.hcurl, a.lHc, a.lE, a.wh, a.has_cpml);
+    else
+        hipLaunchKernelGGL(k_adj_h<2>, grid, dim3(256), 0, st, g, a.ca, a.hcurl, a.lHc, a.lE, a.wh, a.has_cpml);
 }
 
 }  // namespace dgx

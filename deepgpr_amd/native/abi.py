@@ -126,6 +126,9 @@ NATIVE_FUNCTIONS: Dict[str, Tuple[tuple, object]] = {
                              ("count", LONGLONG), ("storage_kind", INT),
                              ("conversion_backend", INT), ("stream", VOID_PTR)), INT),
     "dgx_scratch_bytes": ((("solve", VOID_PTR), ("backward", INT)), LONGLONG),
+    "dgx_set_kernel_timing": ((("enable", INT),), None),
+    "dgx_kernel_timing": ((("kernel", INT), ("total_ms", VOID_PTR), ("launches", VOID_PTR)), INT),
+    "dgx_kernel_name": ((("kernel", INT),), CHAR_PTR),
     "deepgpr_supports_error_reporting": ((), INT),
     "deepgpr_last_error": ((), CHAR_PTR),
     "deepgpr_clear_last_error": ((), None),

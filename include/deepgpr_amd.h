@@ -114,6 +114,17 @@ DGX_API int dgx_codec_roundtrip(const float* input, void* encoded, float* decode
  * admission by HBM footprint. No counterpart in the reference. */
 DGX_API long long dgx_scratch_bytes(const dgx_solve* solve, int backward);
 
+/* Per-launch kernel timing with HIP events on the launch stream, for
+ * measurement only (no counterpart in the reference). enable=1 clears the
+ * totals and starts recording; every later dgx_forward/dgx_backward adds the
+ * elapsed time of each launch to its kernel's total; enable=0 stops
+ * recording and keeps the totals readable. */
+DGX_API void dgx_set_kernel_timing(int enable);
+/* Total milliseconds and launch count recorded for one kernel id. */
+DGX_API int dgx_kernel_timing(int kernel, double* total_ms, long long* launches);
+/* Name of a kernel id ("" past the last id). */
+DGX_API const char* dgx_kernel_name(int kernel);
+
 /* ---------------------------------------------------------------------- */
 /* Reference ABI-6 export set (src/DeepGPR/lib/deepgpr.h:48-152)          */
 /* ---------------------------------------------------------------------- */

@@ -101,3 +101,44 @@ def test_gpu_codec_backends_agree(cuda_device):
         else:
             ref = torch.from_numpy(vals[finite]).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
             assert np.array_equal(out[0][finite], ref)
+
+
+TM2D = 1 << 6
+INACTIVE_PHI = (0, 3, 4, 7, 8, 11, 12, 15)
+
+
+def _tm2d_problem(h, NX, NY, order, seed, nsrc=2, nrx=4):
+    p = h.make_problem(NX, NY, 2, order=order, nt=13, nshot=2, nsrc=nsrc, nrx=nrx, seed=seed,
+                       pml=(4, 3, 3, 4, 0, 0))
+    # the 2D TM contract: Ex, Ey, Hz and their CPML auxiliaries start at zero
+    for arrs, idx in ((p.E0, (0, 1)), (p.H0, (2,)), (p.lam0, (0, 1, 5))):
+        for k in idx:
+            arrs[k] = np.zeros_like(arrs[k])
+    for k in INACTIVE_PHI:
+        p.phi0[k] = np.zeros_like(p.phi0[k])
+        p.lam0[6 + k] = np.zeros_like(p.lam0[6 + k])
+    return p
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+@pytest.mark.parametrize("mode,S,st", [(2, 1, "e_only"), (3, 1, "e_only"), (3, 2, "f32"), (3, 1, "bf16")])
+def test_tm2d_fast_path_matches_general_oracle(oracle_built, cuda_device, order, mode, S, st):
+    h = oracle_built
+    print(f"[tm2d] order={order} mode={mode} S={S} {st}", flush=True)
+    p = _tm2d_problem(h, 26, 23, order, seed=40 + order)
+    p.mode, p.S = mode, S
+    p.storage_type = _storage(h, st)
+    cpu = h.run_cpu(p, "oracle")
+    p.storage_type |= TM2D
+    gpu = h.run_gpu(p, api="dgx")
+    bad = h.bitwise_diff(gpu, cpu, h.KEYS_FWD + ("g_src", "g_eps", "g_sig"))
+    assert not bad, bad
+    # adjoint states: the TM2D contract transposes only Ez, Hx, Hy and their
+    # CPML auxiliaries and leaves the other cotangents untouched (deepgpr.h:77-83)
+    active = [2, 3, 4] + [6 + k for k in range(24) if k not in INACTIVE_PHI]
+    for k in active:
+        assert np.array_equal(gpu["L"][k], cpu["L"][k]), k
+    for k in set(range(30)) - set(active):
+        assert np.array_equal(gpu["L"][k], p.lam0[k]), k
+    assert np.array_equal(gpu["E_saved"], cpu["E_saved"])
+    assert np.array_equal(gpu["R_saved"], cpu["R_saved"])
