@@ -39,6 +39,8 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run a plain device copy of this many MiB first (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--cpu-baseline-child", default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -216,6 +218,14 @@ def main():
     from deepgpr_amd.native.loader import get_library
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration_mb:
+        n = args.copy_calibration_mb * (1 << 20) // 4
+        a = torch.ones(n, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize(dev)
+        print(f"[bench] copy calibration: {args.copy_calibration_mb} MiB read + written", file=sys.stderr, flush=True)
+        del a, b
     w = _workload(args)
     kw = _compute_kwargs(w, args)
     eps = w.eps_r.to(dev)
