@@ -89,6 +89,39 @@ def _psi_elems(w):
     return e, h
 
 
+def _history_terms(w, args):
+    comps = 3 if w.mode == 3 else 1
+    hist_b = {"float32": 4, "bfloat16": 2, "float16": 2, "int8": 1}[args.storage]
+    e_only = args.rhs == "reconstructed" or (args.rhs == "auto" and args.storage == "float32"
+                                             and w.sampling_interval == 1)
+    frames = 1 if e_only else 2  # E^n (+ R^n) per saved step; E-only frames are each read once
+    return comps, hist_b, frames
+
+
+def _direction_bytes(w, args, adjoint: bool) -> float:
+    """Compulsory HBM bytes of one time step of one direction, all shots:
+    every field read and written once, each coefficient plane read once (it is
+    shared by all shots), the active CPML auxiliaries read and written once,
+    the history frames written (forward) or read (adjoint), and the gradient
+    accumulators read and written once per saved step."""
+    shots = w.source_location.shape[0]
+    cells = w.extended_cells()
+    two_d = len(w.physical_shape) == 2
+    pe, ph = (x * shots for x in _psi_elems(w))
+    comps, hist_b, frames = _history_terms(w, args)
+    fields = 3 if two_d else 6
+    planes = 4 if two_d else 5  # ca, ccurl, cb, hcurl (+ hrhs in 3D where every cell may be CPML)
+    total = shots * cells * fields * 8 + cells * planes * 4 + 8 * (pe + ph)
+    S = w.sampling_interval
+    if not adjoint:
+        total += shots * cells * comps * hist_b * frames / S
+    else:
+        read_comps = 1 if two_d else comps  # the 2D TM gradient reads only the Ez component
+        acc = 8 if args.accumulator == "float64" else 4
+        total += (shots * cells * read_comps * hist_b * frames + cells * (4 + 2 * 2 * acc)) / S
+    return total
+
+
 def kernel_bytes(kernel: str, w, args) -> float:
     """Algorithmic HBM bytes of ONE launch of a kernel (one time step, all shots):
     the compulsory loads/stores of that kernel's function, each array element
@@ -96,17 +129,20 @@ def kernel_bytes(kernel: str, w, args) -> float:
     shots = w.source_location.shape[0]
     cells = w.extended_cells() * shots
     two_d = len(w.physical_shape) == 2
-    comps = 3 if w.mode == 3 else 1
     pe, ph = (x * shots for x in _psi_elems(w))
-    hist_b = {"float32": 4, "bfloat16": 2, "float16": 2, "int8": 1}[args.storage]
-    e_only = args.rhs == "reconstructed" or (args.rhs == "auto" and args.storage == "float32" and w.sampling_interval == 1)
+    comps, hist_b, frames = _history_terms(w, args)
+    if kernel == "tm_step_fwd":
+        return _direction_bytes(w, args, adjoint=False)
+    if kernel == "tm_step_adj":
+        # the saved-step share of the material gradient is in tm_grad_reduce
+        acc = 8 if args.accumulator == "float64" else 4
+        return _direction_bytes(w, args, adjoint=True) - w.extended_cells() * (4 + 2 * 2 * acc) / w.sampling_interval
     if kernel == "fwd_h":
         core = 24 if two_d else 40      # E read, H read+write, ch_curl
         return cells * core + 8 * ph
     if kernel == "fwd_e":
         core = 24 if two_d else 48      # H read, E read+write, ca, ce_curl (+ce_rhs in 3D)
-        per_save = comps * hist_b * (1 if e_only else 2) / w.sampling_interval
-        return cells * (core + per_save) + 8 * pe
+        return cells * (core + comps * hist_b * frames / w.sampling_interval) + 8 * pe
     if kernel == "adj_e":
         core = 24 if two_d else 40      # lambda_E read, ce_curl, lambda_H read+write
         return cells * core + 4 * pe
@@ -122,11 +158,8 @@ def kernel_bytes(kernel: str, w, args) -> float:
 
 
 def step_bytes(w, args) -> float:
-    """Algorithmic bytes of one fwd+adj cell-update pair summed over the step's kernels."""
-    k = ("fwd_h", "fwd_e", "adj_e", "adj_h")
-    total = sum(kernel_bytes(n, w, args) for n in k)
-    total += kernel_bytes("material_gradient", w, args) / w.sampling_interval
-    return total
+    """Algorithmic bytes of one forward + one adjoint time step (all shots)."""
+    return _direction_bytes(w, args, adjoint=False) + _direction_bytes(w, args, adjoint=True)
 
 
 # ----------------------------------------------------------------------------- CPU baseline

@@ -2,8 +2,10 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "deepgpr_amd.h"
@@ -61,7 +63,8 @@ double g_timing_ms[kKCount];
 long long g_timing_count[kKCount];
 const char* const kKernelNames[kKCount] = {
     "coefficients", "fwd_h", "fwd_e", "sample", "quant_int8", "save_frame", "adj_receivers",
-    "adj_source", "material_gradient", "adj_pml_e", "adj_e", "adj_pml_h", "adj_h"};
+    "adj_source", "material_gradient", "adj_pml_e", "adj_e", "adj_pml_h", "adj_h", "tm_step_fwd",
+    "tm_step_adj", "tm_grad_reduce", "tm_src_reduce"};
 
 struct LaunchTimer {
     bool on;
@@ -390,29 +393,60 @@ bool tm2d_applies(const dgx_solve* P, const Plan& pl, const std::vector<int>& sl
     return true;
 }
 
-// Compact buffers of the 2D TM path: 3 fields (nshot * plane) and 5
-// coefficient planes.
+// The split per-operator TM kernels stay selectable for comparison
+// (DEEPGPR_AMD_TM2D_KERNELS=split); the fused step kernels are the default.
+bool tm2d_fused()
+{
+    static const bool fused = [] {
+        const char* v = std::getenv("DEEPGPR_AMD_TM2D_KERNELS");
+        return !(v && std::strcmp(v, "split") == 0);
+    }();
+    return fused;
+}
+
+Tm2dGrid tm_grid_of(const Plan& pl)
+{
+    Tm2dGrid g{};
+    g.NX = pl.g.N[0];
+    g.NY = pl.g.N[1];
+    g.plane = g.NX * g.NY;
+    g.tiles = (g.plane + 255) / 256;
+    g.ntj = (g.NY + kTmBJ - 1) / kTmBJ;
+    g.ftiles = ((g.NX + kTmBI - 1) / kTmBI) * g.ntj;
+    g.nshot = pl.g.nshot;
+    for (int a = 0; a < 3; ++a) g.d[a] = pl.g.d[a];
+    for (int f4 = 0; f4 < 4; ++f4) g.pml[f4] = pl.g.pml[f4];
+    g.has_cpml = pl.has_cpml;
+    return g;
+}
+
+int tm_tile_of(const Tm2dGrid& g, int i, int j) { return (i / kTmBI) * g.ntj + j / kTmBJ; }
+int tm_local(int i, int j) { return (i % kTmBI) * kTmBJ + j % kTmBJ; }
+
+// Compact buffers of the 2D TM path: 3 fields (nshot * plane; 6 with the
+// fused kernels' ping-pong copies) and 5 coefficient planes, plus the second
+// copies of the 8 active CPML auxiliaries.
 struct Tm2dState {
     Tm2dGrid g;
     Tm2dFields f;
-    DevBuf buf;
-    int setup(const Plan& pl, const dgx_solve* P, hipStream_t stream)
+    TmPingPong pp{};
+    bool fused = false;
+    float* home_e[4] = {};
+    float* home_h[4] = {};
+    long long psi_len_e[4] = {}, psi_len_h[4] = {};
+    DevBuf buf, alt;
+    int setup(const Plan& pl, const dgx_solve* P, hipStream_t stream, bool use_fused)
     {
-        g.NX = pl.g.N[0];
-        g.NY = pl.g.N[1];
-        g.plane = g.NX * g.NY;
-        g.tiles = (g.plane + 255) / 256;
-        g.nshot = pl.g.nshot;
-        for (int a = 0; a < 3; ++a) g.d[a] = pl.g.d[a];
-        for (int f4 = 0; f4 < 4; ++f4) g.pml[f4] = pl.g.pml[f4];
-        g.has_cpml = pl.has_cpml;
+        fused = use_fused;
+        g = tm_grid_of(pl);
         const long long field = (long long)g.nshot * g.plane;
-        if (!buf.alloc(((size_t)3 * field + 5 * (size_t)g.plane) * sizeof(float))) return -1;
+        const int nf = fused ? 6 : 3;
+        if (!buf.alloc(((size_t)nf * field + 5 * (size_t)g.plane) * sizeof(float))) return -1;
         float* b = (float*)buf.p;
         f.Ez = b;
         f.Hx = b + field;
         f.Hy = b + 2 * field;
-        float* co = b + 3 * field;
+        float* co = b + nf * field;
         const float* full[5] = {P->update_coeff[0], P->update_coeff[1], P->update_coeff[2], P->update_coeff[4],
                                 P->update_coeff[5]};
         for (int c = 0; c < 5; ++c) launch_tm_pack(full[c], co + (long long)c * g.plane, g.plane, false, stream);
@@ -421,25 +455,118 @@ struct Tm2dState {
         f.cb = co + 2 * (long long)g.plane;
         f.hcurl = co + 3 * (long long)g.plane;
         f.hrhs = co + 4 * (long long)g.plane;
+        long long alt_len = 0;
         for (int f4 = 0; f4 < 4; ++f4) {
             f.coef_e[f4] = P->pml_coeff[f4];
             f.coef_h[f4] = P->pml_coeff[6 + f4];
             f.psi_e[f4] = P->phi[4 * f4 + 1];
             f.psi_h[f4] = P->phi[4 * f4 + 2];
+            home_e[f4] = f.psi_e[f4];
+            home_h[f4] = f.psi_h[f4];
+            const long long across = (long long)g.nshot * (f4 < 2 ? g.NY : g.NX);
+            psi_len_e[f4] = g.pml[f4] > 0 ? across * (g.pml[f4] + 1) : 0;
+            psi_len_h[f4] = g.pml[f4] > 0 ? across * g.pml[f4] : 0;
+            alt_len += psi_len_e[f4] + psi_len_h[f4];
         }
         launch_tm_pack(P->E[2], f.Ez, field, false, stream);
         launch_tm_pack(P->H[0], f.Hx, field, false, stream);
         launch_tm_pack(P->H[1], f.Hy, field, false, stream);
+        if (!fused) return 0;
+        pp.Ez[0] = f.Ez;
+        pp.Hx[0] = f.Hx;
+        pp.Hy[0] = f.Hy;
+        pp.Ez[1] = b + 3 * field;
+        pp.Hx[1] = b + 4 * field;
+        pp.Hy[1] = b + 5 * field;
+        if (alt_len > 0 && !alt.alloc((size_t)alt_len * sizeof(float))) return -1;
+        float* a = (float*)alt.p;
+        // The second copies start equal to the first: auxiliaries no cell
+        // updates (the deepest electric layer) keep their value in both.
+        for (int f4 = 0; f4 < 4; ++f4) {
+            pp.psi_e[0][f4] = home_e[f4];
+            pp.psi_h[0][f4] = home_h[f4];
+            pp.psi_e[1][f4] = psi_len_e[f4] ? a : nullptr;
+            if (psi_len_e[f4] &&
+                hipMemcpyAsync(a, home_e[f4], psi_len_e[f4] * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+                return -1;
+            a += psi_len_e[f4];
+            pp.psi_h[1][f4] = psi_len_h[f4] ? a : nullptr;
+            if (psi_len_h[f4] &&
+                hipMemcpyAsync(a, home_h[f4], psi_len_h[f4] * 4, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+                return -1;
+            a += psi_len_h[f4];
+        }
         return 0;
     }
-    void unpack(const dgx_solve* P, hipStream_t stream)
+    // after a fused step: what was written becomes what is read
+    void flip()
+    {
+        std::swap(pp.Ez[0], pp.Ez[1]);
+        std::swap(pp.Hx[0], pp.Hx[1]);
+        std::swap(pp.Hy[0], pp.Hy[1]);
+        for (int f4 = 0; f4 < 4; ++f4) {
+            std::swap(pp.psi_e[0][f4], pp.psi_e[1][f4]);
+            std::swap(pp.psi_h[0][f4], pp.psi_h[1][f4]);
+        }
+    }
+    int unpack(const dgx_solve* P, hipStream_t stream)
     {
         const long long field = (long long)g.nshot * g.plane;
-        launch_tm_pack(P->E[2], f.Ez, field, true, stream);
-        launch_tm_pack(P->H[0], f.Hx, field, true, stream);
-        launch_tm_pack(P->H[1], f.Hy, field, true, stream);
+        const float* ez = fused ? pp.Ez[0] : f.Ez;
+        const float* hx = fused ? pp.Hx[0] : f.Hx;
+        const float* hy = fused ? pp.Hy[0] : f.Hy;
+        launch_tm_pack(P->E[2], (float*)ez, field, true, stream);
+        launch_tm_pack(P->H[0], (float*)hx, field, true, stream);
+        launch_tm_pack(P->H[1], (float*)hy, field, true, stream);
+        if (!fused) return 0;
+        for (int f4 = 0; f4 < 4; ++f4) {
+            if (psi_len_e[f4] && pp.psi_e[0][f4] != home_e[f4] &&
+                hipMemcpyAsync(home_e[f4], pp.psi_e[0][f4], psi_len_e[f4] * 4, hipMemcpyDeviceToDevice, stream) !=
+                    hipSuccess)
+                return -1;
+            if (psi_len_h[f4] && pp.psi_h[0][f4] != home_h[f4] &&
+                hipMemcpyAsync(home_h[f4], pp.psi_h[0][f4], psi_len_h[f4] * 4, hipMemcpyDeviceToDevice, stream) !=
+                    hipSuccess)
+                return -1;
+        }
+        return 0;
     }
 };
+
+// Source entries of the fused TM kernels grouped by owning (tile, shot):
+// (cell index inside the tile, source index), sources in index order.
+void tm_source_lists(const Tm2dGrid& g, const std::vector<int>& sloc, int nsrc, std::vector<int>& off,
+                     std::vector<int2>& ents)
+{
+    const long long nblk = (long long)g.ftiles * g.nshot;
+    off.assign((size_t)nblk + 1, 0);
+    ents.clear();
+    std::vector<std::pair<long long, int2>> tmp;
+    for (int s = 0; s < g.nshot; ++s)
+        for (int k = 0; k < nsrc; ++k) {
+            const int* L = &sloc[((size_t)s * nsrc + k) * 3];
+            tmp.push_back({(long long)tm_tile_of(g, L[0], L[1]) + (long long)g.ftiles * s,
+                           make_int2(tm_local(L[0], L[1]), k)});
+        }
+    std::stable_sort(tmp.begin(), tmp.end(),
+                     [](const std::pair<long long, int2>& a, const std::pair<long long, int2>& b) {
+                         return a.first < b.first;
+                     });
+    for (const auto& e : tmp) {
+        off[(size_t)e.first + 1] += 1;
+        ents.push_back(e.second);
+    }
+    for (size_t b = 1; b < off.size(); ++b) off[b] += off[b - 1];
+}
+
+// Device copy of a host-built list (nothing to do for an empty one).
+template <typename T>
+bool upload(DevBuf& d, const std::vector<T>& v, hipStream_t st)
+{
+    if (v.empty()) return true;
+    return d.alloc(v.size() * sizeof(T)) &&
+           hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) == hipSuccess;
+}
 
 int run_forward(const dgx_solve* P, hipStream_t stream)
 {
@@ -488,11 +615,15 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
 
     const bool inject = P->nsource > 0 && P->source_component >= 0 && P->source_component <= 2;
     const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
+    const bool fused = tm && tm2d_fused();
+    const Tm2dGrid tg = tm_grid_of(pl);
     const int tm_tiles = (g.N[0] * g.N[1] + 255) / 256;
     long long nblocks = tm ? (long long)tm_tiles * g.nshot : (long long)grid.x * grid.y * grid.z;
     std::vector<int> boff((size_t)nblocks + 1, 0);
     std::vector<int2> ents;
-    if (inject) {
+    if (inject && fused) {
+        tm_source_lists(tg, hsloc, P->nsource, boff, ents);
+    } else if (inject) {
         struct E4 {
             long long blk;
             int order;
@@ -540,6 +671,42 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         src.vol = g.d[0] * g.d[1] * g.d[2];
     }
 
+    // --- receivers of the fused 2D TM kernels: grouped by owning (tile, shot)
+    const int rcomp = (P->receiver_component >= 0 && P->receiver_component <= 2) ? P->receiver_component : 2;
+    const bool fused_rx = fused && P->nreceiver > 0 && rcomp == 2;
+    DevBuf d_roff, d_rent;
+    TmReceiversFwd trx{};
+    if (fused_rx) {
+        std::vector<std::pair<long long, int2>> tmp;
+        for (int s = 0; s < g.nshot; ++s)
+            for (int r = 0; r < P->nreceiver; ++r) {
+                const int* L = &hrloc[((size_t)s * P->nreceiver + r) * 3];
+                tmp.push_back({(long long)tm_tile_of(tg, L[0], L[1]) + (long long)tg.ftiles * s,
+                               make_int2(tm_local(L[0], L[1]), r)});
+            }
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const std::pair<long long, int2>& x, const std::pair<long long, int2>& y) {
+                             return x.first < y.first;
+                         });
+        std::vector<int> roff((size_t)tg.ftiles * g.nshot + 1, 0);
+        std::vector<int2> rent;
+        for (const auto& e : tmp) {
+            roff[(size_t)e.first + 1] += 1;
+            rent.push_back(e.second);
+        }
+        for (size_t b = 1; b < roff.size(); ++b) roff[b] += roff[b - 1];
+        if (!upload(d_roff, roff, stream) || !upload(d_rent, rent, stream)) {
+            set_error("forward: out of device memory for receiver lists");
+            return -1;
+        }
+        trx.off = (const int*)d_roff.p;
+        trx.ent = (const int2*)d_rent.p;
+        trx.out = P->receiver_data;
+        trx.nrx = P->nreceiver;
+        trx.nt = nt;
+        trx.count = (int)rent.size();
+    }
+
     // --- history buffers -----------------------------------------------------
     const bool int8 = pl.kind == kInt8;
     if (pl.history && (!P->E_saved || ((pl.store_r || pl.e_only) && !P->R_saved))) {
@@ -575,9 +742,8 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     a.has_cpml = pl.has_cpml;
     a.src = src;
     const size_t eb = elem_bytes(pl.kind);
-    const int rcomp = (P->receiver_component >= 0 && P->receiver_component <= 2) ? P->receiver_component : 2;
     Tm2dState tms;
-    if (tm && tms.setup(pl, P, stream)) {
+    if (tm && tms.setup(pl, P, stream, fused)) {
         set_error("forward: out of device memory for the 2D TM state");
         return -1;
     }
@@ -614,7 +780,11 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         }
         a.hist = hw;
         a.src.it = it;
-        if (tm) {
+        if (fused) {
+            TIMED(tmr, kKTmStepFwd,
+                  launch_tm_step_fwd(pl.order, tms.g, tms.f, tms.pp, a.src, trx, a.hist, it, stream));
+            tms.flip();
+        } else if (tm) {
             TIMED(tmr, kKFwdH, launch_tm_fwd_h(pl.order, tms.g, tms.f, stream));
             TIMED(tmr, kKFwdE, launch_tm_fwd_e(pl.order, tms.g, tms.f, a.src, a.hist, stream));
         } else {
@@ -625,7 +795,7 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
             TIMED(tmr, kKQuant,
                   launch_quant_int8(g, a.Ec, (const float*)d_rscratch.p, pl.snap,
                                     int8_writer(pl, P->R_saved, pl.nt_saved, ts), stream));
-        if (P->nreceiver > 0) {
+        if (P->nreceiver > 0 && !fused_rx) {
             if (tm && rcomp == 2)
                 TIMED(tmr, kKSample,
                       launch_sample(g, tms.f.Ez, P->receiver_location, P->receiver_data, P->nreceiver, nt, it,
@@ -636,7 +806,10 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
                                     stream));
         }
     }
-    if (tm) tms.unpack(P, stream);
+    if (tm && tms.unpack(P, stream)) {
+        set_error("forward: copy back of the 2D TM state");
+        return -1;
+    }
     if (pl.e_only) {
         if (int8) {
             launch_quant_int8(g, a.Ec, nullptr, 0, int8_writer(pl, P->R_saved, 1, 0), stream);
@@ -654,6 +827,35 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     DGX_TRY(hipStreamSynchronize(stream), "forward: stream synchronize");
     tmr.collect();
     return 0;
+}
+
+// History frames and weight of the saved step that starts at time step it.
+MatGrad mat_step(const Plan& pl, const dgx_solve* P, MatGrad mg, int it, int nt, int S)
+{
+    const size_t eb = elem_bytes(pl.kind);
+    int wgt = S;
+    const int ts = it / S;
+    if (it + wgt > nt) wgt = nt - it;
+    mg.weight = wgt;
+    if (pl.kind == kInt8) {
+        mg.e8 = int8_view(pl, P->E_saved, pl.nt_saved, ts);
+        if (!pl.e_only) mg.s8 = int8_view(pl, P->R_saved, pl.nt_saved, ts);
+        else if (ts + 1 < pl.nt_saved) mg.s8 = int8_view(pl, P->E_saved, pl.nt_saved, ts + 1);
+        else mg.s8 = int8_view(pl, P->R_saved, 1, 0);
+    } else {
+        mg.eframe = (const unsigned char*)P->E_saved + (size_t)ts * pl.snap * eb;
+        mg.e_cstride = pl.cstride;
+        mg.s_cstride = pl.cstride;
+        if (!pl.e_only) {
+            mg.sframe = (const unsigned char*)P->R_saved + (size_t)ts * pl.snap * eb;
+        } else if (ts + 1 < pl.nt_saved) {
+            mg.sframe = (const unsigned char*)P->E_saved + (size_t)(ts + 1) * pl.snap * eb;
+        } else {
+            mg.sframe = P->R_saved;
+            mg.s_cstride = pl.snap;
+        }
+    }
+    return mg;
 }
 
 int run_backward(const dgx_solve* P, hipStream_t stream)
@@ -712,6 +914,8 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     }
 
     const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
+    const bool fused = tm && tm2d_fused();
+    const Tm2dGrid tg = tm_grid_of(pl);
     Grid gc = g;  // compact 2D TM addressing for the receiver/source kernels
     if (tm) {
         gc.st0 = g.N[1];
@@ -766,11 +970,83 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
         rg.ngroups = (int)gshot.size();
     }
 
+    // fused 2D TM kernels: the groups again, listed for every (tile, shot)
+    // whose staged region (tile plus halo) holds the cell
+    DevBuf d_xoff, d_xgrp;
+    TmReceiversAdj tax{};
+    if (fused && adj_rx && !gshot.empty()) {
+        const int he = tm_halo(pl.order);
+        const int lc = kTmBJ + 2 * he;
+        const int nti = (tg.NX + kTmBI - 1) / kTmBI;
+        std::vector<std::pair<long long, int4>> tmp;
+        for (size_t q = 0; q < gshot.size(); ++q) {
+            const int i = (int)(gcell[q] / tg.NY), j = (int)(gcell[q] % tg.NY);
+            for (int ti = std::max(0, (i - he) / kTmBI - 1); ti <= std::min(nti - 1, (i + he) / kTmBI); ++ti) {
+                const int r = i - (ti * kTmBI - he);
+                if (r < 0 || r >= kTmBI + 2 * he) continue;
+                for (int tj = std::max(0, (j - he) / kTmBJ - 1); tj <= std::min(tg.ntj - 1, (j + he) / kTmBJ);
+                     ++tj) {
+                    const int c = j - (tj * kTmBJ - he);
+                    if (c < 0 || c >= lc) continue;
+                    tmp.push_back({(long long)(ti * tg.ntj + tj) + (long long)tg.ftiles * gshot[q],
+                                   make_int4(r * lc + c, goff[q], goff[q + 1] - goff[q], 0)});
+                }
+            }
+        }
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const std::pair<long long, int4>& x, const std::pair<long long, int4>& y) {
+                             return x.first < y.first;
+                         });
+        std::vector<int> xoff((size_t)tg.ftiles * g.nshot + 1, 0);
+        std::vector<int4> xgrp;
+        for (const auto& e : tmp) {
+            xoff[(size_t)e.first + 1] += 1;
+            xgrp.push_back(e.second);
+        }
+        for (size_t k = 1; k < xoff.size(); ++k) xoff[k] += xoff[k - 1];
+        if (!upload(d_xoff, xoff, stream) || !upload(d_xgrp, xgrp, stream)) {
+            set_error("backward: out of device memory for receiver lists");
+            return -1;
+        }
+        tax.off = (const int*)d_xoff.p;
+        tax.grp = (const int4*)d_xgrp.p;
+        tax.member = rg.member;
+        tax.dgrad = P->data_grad;
+        tax.nrx = P->nreceiver;
+        tax.nt = nt;
+        tax.count = (int)xgrp.size();
+    }
+    // fused 2D TM kernels: source lists, per-shot source and material terms
+    DevBuf d_soff, d_sent, d_svals, d_terms;
+    SourceBlocks tsrc{};
+    TmAdjGrad tgrad{};
+    if (fused && want_src) {
+        std::vector<int> soff;
+        std::vector<int2> sent;
+        tm_source_lists(tg, hsloc, P->nsource, soff, sent);
+        if (!upload(d_soff, soff, stream) || !upload(d_sent, sent, stream) ||
+            !d_svals.alloc((size_t)nt * g.nshot * P->nsource * sizeof(float))) {
+            set_error("backward: out of device memory for source lists");
+            return -1;
+        }
+        tsrc.block_off = (const int*)d_soff.p;
+        tsrc.entries = (const int2*)d_sent.p;
+        tsrc.count = (int)sent.size();
+        tgrad.src_vals = (float*)d_svals.p;
+        tgrad.nsrc = P->nsource;
+    }
+    if (fused && want_mat && !d_terms.alloc((size_t)2 * g.nshot * pl.box.cells * sizeof(float))) {
+        set_error("backward: out of device memory for per-shot gradient terms");
+        return -1;
+    }
+
     // CPML weight scratch, shaped like the psi arrays
     DevBuf d_w;
     Cpml we{}, wh{};
     Tm2dWeights tw{};
-    if (pl.has_cpml && tm) {
+    if (pl.has_cpml && fused) {
+        // the fused kernels keep the weights in LDS
+    } else if (pl.has_cpml && tm) {
         long long tot = 0;
         for (int f = 0; f < 4; ++f) {
             long long across = f < 2 ? g.N[1] : g.N[0];
@@ -857,18 +1133,34 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     mg.want_sig = P->sigma_requires_grad == 1;
     mg.dt = P->dt;
     mg.box = pl.box;
-    const size_t eb = elem_bytes(pl.kind);
     const float geo = want_src ? g.d[P->source_component] / (g.d[0] * g.d[1] * g.d[2]) : 0.0f;
     Tm2dState tms;
-    if (tm && tms.setup(pl, P, stream)) {
+    if (tm && tms.setup(pl, P, stream, fused)) {
         set_error("backward: out of device memory for the 2D TM state");
         return -1;
     }
+    tgrad.geo = geo;
+    tgrad.ez_comp = pl.comps == 3 ? 2 : 0;
     float* lam_rx = tm ? tms.f.Ez : (adj_rx ? P->E[P->receiver_component] : nullptr);
     float* lam_src = tm ? tms.f.Ez : (want_src ? P->E[P->source_component] : nullptr);
     const float* cb_src = tm ? tms.f.cb : co[2];
 
     for (int it = nt - 1; it >= 0; --it) {
+        if (fused) {
+            tgrad.terms = nullptr;
+            if (want_mat && it % S == 0) {
+                tgrad.mg = mat_step(pl, P, mg, it, nt, S);
+                tgrad.terms = (float*)d_terms.p;
+            }
+            TIMED(tmr, kKTmStepAdj,
+                  launch_tm_step_adj(pl.order, tms.g, tms.f, tms.pp, tsrc, tax, tgrad, it, stream));
+            tms.flip();
+            if (tgrad.terms)
+                TIMED(tmr, kKTmGradReduce,
+                      launch_tm_grad_reduce(tms.g, pl.box, tgrad.terms, mg.want_eps, mg.want_sig, geps, gsig, fp64,
+                                            stream));
+            continue;
+        }
         if (adj_rx)
             TIMED(tmr, kKAdjRx, launch_adj_receivers(gc, rg, lam_rx, P->data_grad, P->nreceiver, nt, it, stream));
         if (want_src)
@@ -876,28 +1168,7 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
                   launch_adj_source(gc, P->source_location, cb_src, lam_src, P->grad_source, P->nsource, nt, it,
                                     geo, stream));
         if (want_mat && it % S == 0) {
-            int wgt = S;
-            const int ts = it / S;
-            if (it + wgt > nt) wgt = nt - it;
-            mg.weight = wgt;
-            if (pl.kind == kInt8) {
-                mg.e8 = int8_view(pl, P->E_saved, pl.nt_saved, ts);
-                if (!pl.e_only) mg.s8 = int8_view(pl, P->R_saved, pl.nt_saved, ts);
-                else if (ts + 1 < pl.nt_saved) mg.s8 = int8_view(pl, P->E_saved, pl.nt_saved, ts + 1);
-                else mg.s8 = int8_view(pl, P->R_saved, 1, 0);
-            } else {
-                mg.eframe = (const unsigned char*)P->E_saved + (size_t)ts * pl.snap * eb;
-                mg.e_cstride = pl.cstride;
-                mg.s_cstride = pl.cstride;
-                if (!pl.e_only) {
-                    mg.sframe = (const unsigned char*)P->R_saved + (size_t)ts * pl.snap * eb;
-                } else if (ts + 1 < pl.nt_saved) {
-                    mg.sframe = (const unsigned char*)P->E_saved + (size_t)(ts + 1) * pl.snap * eb;
-                } else {
-                    mg.sframe = P->R_saved;
-                    mg.s_cstride = pl.snap;
-                }
-            }
+            mg = mat_step(pl, P, mg, it, nt, S);
             if (tm)
                 TIMED(tmr, kKMatGrad,
                       launch_tm_material_gradient(tms.g, tms.f, mg, pl.comps == 3 ? 2 : 0, geps, gsig, fp64, stream));
@@ -916,7 +1187,13 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             TIMED(tmr, kKAdjH, launch_adj_h(pl.order, g, a, stream));
         }
     }
-    if (tm) tms.unpack(P, stream);
+    if (tm && tms.unpack(P, stream)) {
+        set_error("backward: copy back of the 2D TM state");
+        return -1;
+    }
+    if (fused && want_src)
+        TIMED(tmr, kKTmSrcReduce,
+              launch_tm_src_reduce(tgrad.src_vals, P->grad_source, g.nshot, P->nsource, nt, stream));
     if (want_mat && fp64) {
         if (P->eps_r_requires_grad == 1) launch_add_f64((const double*)geps, P->grad_eps_r, ngrad, stream);
         if (P->sigma_requires_grad == 1) launch_add_f64((const double*)gsig, P->grad_sigma, ngrad, stream);

@@ -85,8 +85,16 @@ __device__ __forceinline__ int fit_fwd(int c, int n, int want)
 }
 
 // sum_r c_r (f[c + r - 1] - f[c - r]) with the radius clamped at the faces.
-template <int ORDER>
-__device__ __forceinline__ float dbwd(const float* __restrict__ f, long long id, long long st, int c, int n)
+// IDX is int for LDS tiles (32-bit address math) and long long for global
+// fields; it is deduced from the index alone (the stride is converted to it).
+template <typename T>
+struct NoDeduce {
+    using type = T;
+};
+
+template <int ORDER, typename IDX = long long>
+__device__ __forceinline__ float dbwd(const float* __restrict__ f, IDX id, typename NoDeduce<IDX>::type st, int c,
+                                      int n)
 {
     if constexpr (ORDER <= 2) {
         return f[id] - f[id - st];
@@ -105,8 +113,9 @@ __device__ __forceinline__ float dbwd(const float* __restrict__ f, long long id,
 }
 
 // sum_r c_r (f[c + r] - f[c - r + 1])
-template <int ORDER>
-__device__ __forceinline__ float dfwd(const float* __restrict__ f, long long id, long long st, int c, int n)
+template <int ORDER, typename IDX = long long>
+__device__ __forceinline__ float dfwd(const float* __restrict__ f, IDX id, typename NoDeduce<IDX>::type st, int c,
+                                      int n)
 {
     if constexpr (ORDER <= 2) {
         return f[id + st] - f[id];

@@ -92,6 +92,8 @@ struct Tm2dGrid {
     int NX, NY;
     int plane;   // NX * NY
     int tiles;   // ceil(plane / 256)
+    int ftiles;  // tiles of the fused step kernels: nti * ntj
+    int ntj;     // kTmBJ-wide tile columns
     int nshot;
     float d[3];
     int pml[4];  // x0, xm, y0, ym
@@ -111,6 +113,71 @@ struct Tm2dWeights {
     float* we[4];
     float* wh[4];
 };
+
+// Fused step kernels (dgx_tm2d_fused.hip): a workgroup owns kTmBI x kTmBJ
+// cells of one shot; fields and the active auxiliaries ping-pong between
+// index 0 (read) and index 1 (written) every time step.
+#ifndef DGX_TM_TILE_ROWS
+#define DGX_TM_TILE_ROWS 16
+#endif
+#ifndef DGX_TM_FWD_THREADS
+#define DGX_TM_FWD_THREADS 512
+#endif
+#ifndef DGX_TM_ADJ_THREADS
+#define DGX_TM_ADJ_THREADS 512
+#endif
+constexpr int kTmBI = DGX_TM_TILE_ROWS;
+constexpr int kTmBJ = 64;  // one wavefront per tile row
+__host__ __device__ constexpr int tm_halo(int order) { return order == 8 ? 7 : (order == 4 ? 3 : 1); }
+
+struct TmPingPong {
+    float* Ez[2];
+    float* Hx[2];
+    float* Hy[2];
+    float* psi_e[2][4];
+    float* psi_h[2][4];
+};
+
+// Receivers of the forward sweep, grouped by owning (tile, shot):
+// off[t] .. off[t+1] index ent = (cell index inside the tile, receiver).
+struct TmReceiversFwd {
+    const int* off;
+    const int2* ent;
+    float* out;  // (nshot, nt, nrx)
+    int nrx, nt, count;
+};
+
+// Receiver groups of the adjoint sweep for every (tile, shot) whose staged
+// region (tile plus halo) holds the cell: grp = (staged index, first member,
+// member count, 0); members are receiver indices in index order.
+struct TmReceiversAdj {
+    const int* off;
+    const int4* grp;
+    const int* member;
+    const float* dgrad;  // (nshot, nt, nrx)
+    int nrx, nt, count;
+};
+
+// Per-shot gradient terms written by the fused adjoint and summed over shots
+// in order afterwards (keeps the FP32 summation order of the unfused path).
+struct TmAdjGrad {
+    float* terms;     // [eps | sig] x (nshot, box cells); null when no material gradient this step
+    float* src_vals;  // (nt, nshot, nsrc); null without a source gradient
+    int nsrc;
+    float geo;
+    int ez_comp;
+    MatGrad mg;
+};
+
+void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                        const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
+                        hipStream_t st);
+void launch_tm_step_adj(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                        const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it,
+                        hipStream_t st);
+void launch_tm_grad_reduce(const Tm2dGrid& g, const Box& b, const float* terms, int want_eps, int want_sig,
+                           void* geps, void* gsig, bool fp64, hipStream_t st);
+void launch_tm_src_reduce(const float* vals, float* gsrc, int nshot, int nsrc, int nt, hipStream_t st);
 
 void launch_tm_fwd_h(int order, const Tm2dGrid& g, const Tm2dFields& f, hipStream_t st);
 void launch_tm_fwd_e(int order, const Tm2dGrid& g, const Tm2dFields& f, const SourceBlocks& src,
@@ -148,7 +215,7 @@ void launch_adj_h(int order, const Grid& g, const AdjArgs& a, hipStream_t st);
 // Kernel ids of the optional per-launch timing (dgx_set_kernel_timing).
 enum KernelId : int {
     kKCoeff = 0, kKFwdH, kKFwdE, kKSample, kKQuant, kKSaveFrame, kKAdjRx, kKAdjSrc, kKMatGrad, kKAdjPmlE,
-    kKAdjE, kKAdjPmlH, kKAdjH, kKCount
+    kKAdjE, kKAdjPmlH, kKAdjH, kKTmStepFwd, kKTmStepAdj, kKTmGradReduce, kKTmSrcReduce, kKCount
 };
 
 }  // namespace dgx

@@ -1,0 +1,711 @@
+// One kernel per 2D-TM time step, forward and adjoint.
+//
+// A workgroup owns a kTmBI x kTmBJ tile of one shot. It stages Ez (or
+// lambda_Ez) on the tile plus a (2R-1)-cell halo in LDS, recomputes the
+// magnetic half step (or the transposed electric weights) on the R-cell halo
+// the electric stencil needs, and writes only the cells it owns. Fields and
+// the active CPML auxiliaries ping-pong between two buffers, so a halo read
+// never races another workgroup's write. Every value uses the expression of
+// the split kernels in dgx_tm2d.hip (the same operations in the same order),
+// so both agree bit for bit; those in turn follow the general path.
+#include "dgx_device.h"
+#include "dgx_internal.h"
+
+namespace dgx {
+
+namespace {
+
+constexpr int BI = kTmBI;
+constexpr int BJ = kTmBJ;
+constexpr int NTF = DGX_TM_FWD_THREADS;  // threads per workgroup, forward
+constexpr int NTA = DGX_TM_ADJ_THREADS;  // threads per workgroup, adjoint
+
+constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+template <int ORDER>
+struct Tile {
+    static constexpr int R = Radius<ORDER>::value;
+    static constexpr int HE = 2 * R - 1;       // halo of the staged Ez / lambda_Ez
+    static constexpr int LR = BI + 2 * HE;     // staged rows
+    static constexpr int LC = BJ + 2 * HE;     // staged columns
+    static constexpr int HR = BI + 2 * R - 1;  // rows of the Hy strip: R above the tile, R-1 below
+    static constexpr int HC = BJ + 2 * R - 1;  // columns of the Hx strip
+    static_assert(HE == tm_halo(ORDER), "halo width");
+};
+
+// Same XCD-aware placement as the split kernels: XCD x takes the x-th
+// contiguous chunk of tiles.
+__device__ __forceinline__ int step_tile(int tiles)
+{
+    const int chunk = gridDim.x >> 3;
+    const int t = (blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+    return t < tiles ? t : -1;
+}
+
+__device__ __forceinline__ int face_q2(const Tm2dGrid& g, bool electric, int face, int c)
+{
+    const int p = g.pml[face];
+    const int n = face < 2 ? g.NX : g.NY;
+    if (p <= 0) return -1;
+    if ((face & 1) == 0) {
+        if (electric) return (c >= 1 && c <= p) ? p - c : -1;
+        return (c >= 0 && c <= p - 1) ? p - 1 - c : -1;
+    }
+    if (c >= n - 1 - p && c <= n - 2) {
+        if (electric && c <= 0) return -1;
+        return c - (n - 1 - p);
+    }
+    return -1;
+}
+
+// x faces (s, q, j), y faces (s, i, q)
+__device__ __forceinline__ long long psi_at(const Tm2dGrid& g, bool electric, int face, int s, int i, int j, int q)
+{
+    const int depth = g.pml[face] + (electric ? 1 : 0);
+    if (face < 2) return ((long long)s * depth + q) * g.NY + j;
+    return ((long long)s * g.NX + i) * depth + q;
+}
+
+__device__ __forceinline__ float axis_k(const Tm2dGrid& g, float c1, int a)
+{
+    if (a == 0 || g.d[a] == g.d[0]) return c1;
+    return c1 * g.d[0] / g.d[a];
+}
+
+__device__ __forceinline__ bool ez_up(const Tm2dGrid& g, int i, int j)
+{
+    return ((g.NX - 1) != 1 || (g.NY - 1) != 1) && i > 0 && i < g.NX - 1 && j > 0 && j < g.NY - 1;
+}
+__device__ __forceinline__ bool hx_up(const Tm2dGrid& g, int i, int j)
+{
+    return (g.NX - 1) != 1 && i > 0 && i < g.NX - 1 && j < g.NY - 1;
+}
+__device__ __forceinline__ bool hy_up(const Tm2dGrid& g, int i, int j)
+{
+    return (g.NY - 1) != 1 && i < g.NX - 1 && j > 0 && j < g.NY - 1;
+}
+
+struct Cpml4 {
+    float ra1, rb, re, rf;
+};
+__device__ __forceinline__ Cpml4 cpml_at(const float* R, int p, int q)
+{
+    return {R[q] - 1.0f, R[p + q], R[2 * p + q], R[3 * p + q]};
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Forward: H half step (tile + halo), E half step, sources, history, receivers
+// ---------------------------------------------------------------------------
+template <int ORDER>
+__global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
+                                                     TmReceiversFwd rx, HistWrite hist, int it)
+{
+    using T = Tile<ORDER>;
+    constexpr int R = T::R, HE = T::HE, LR = T::LR, LC = T::LC, HR = T::HR, HC = T::HC;
+    constexpr int KS = cdiv(LR * LC, NTF), KY = cdiv(HR * BJ, NTF), KX = cdiv(BI * HC, NTF),
+                  KE = cdiv(BI * BJ, NTF);
+    __shared__ float sE[LR * LC];
+    __shared__ float sHy[HR * BJ];
+    __shared__ float sHx[BI * HC];
+    __shared__ float sEn[BI * BJ];
+    const int tile = step_tile(g.ftiles);
+    if (tile < 0) return;
+    const int s = blockIdx.y, tid = threadIdx.x;
+    const int ti = tile / g.ntj, tj = tile - ti * g.ntj;
+    const int i0 = ti * BI, j0 = tj * BJ;
+    const long long sb0 = (long long)s * g.plane;
+    const float* __restrict__ Ez = pp.Ez[0] + sb0;
+    const float* __restrict__ Hx = pp.Hx[0] + sb0;
+    const float* __restrict__ Hy = pp.Hy[0] + sb0;
+
+    // Every load that does not depend on LDS is issued before the first
+    // barrier, so a workgroup waits for memory about once per phase set.
+    float st[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        const int e = tid + k * NTF;
+        const int r = e / LC, c = e - (e / LC) * LC;
+        const int gi = i0 - HE + r, gj = j0 - HE + c;
+        st[k] = (e < LR * LC && gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY) ? Ez[gi * g.NY + gj] : 0.0f;
+    }
+    float y_h[KY], y_u[KY];  // Hy and hcurl of the Hy strip
+#pragma unroll
+    for (int k = 0; k < KY; ++k) {
+        const int e = tid + k * NTF;
+        const int gi = i0 - R + e / BJ, gj = j0 + e % BJ;
+        const bool ok = e < HR * BJ && gi >= 0 && gi < g.NX && gj < g.NY;
+        y_h[k] = ok ? Hy[gi * g.NY + gj] : 0.0f;
+        y_u[k] = ok ? f.hcurl[gi * g.NY + gj] : 0.0f;
+    }
+    float x_h[KX], x_u[KX];  // Hx and hcurl of the Hx strip
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + k * NTF;
+        const int gi = i0 + e / HC, gj = j0 - R + e % HC;
+        const bool ok = e < BI * HC && gi < g.NX && gj >= 0 && gj < g.NY;
+        x_h[k] = ok ? Hx[gi * g.NY + gj] : 0.0f;
+        x_u[k] = ok ? f.hcurl[gi * g.NY + gj] : 0.0f;
+    }
+    float t_ca[KE], t_cc[KE], t_cb[KE];  // coefficients of the tile cells
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int e = tid + k * NTF;
+        const int gi = i0 + e / BJ, gj = j0 + e % BJ;
+        const bool ok = e < BI * BJ && gi < g.NX && gj < g.NY;
+        t_ca[k] = ok ? f.ca[gi * g.NY + gj] : 0.0f;
+        t_cc[k] = ok ? f.ccurl[gi * g.NY + gj] : 0.0f;
+        t_cb[k] = ok ? f.cb[gi * g.NY + gj] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+        if (tid + k * NTF < LR * LC) sE[tid + k * NTF] = st[k];
+    __syncthreads();
+
+    // Hy on rows [i0-R, i0+BI+R-1) of the tile's columns (x faces: H phi1)
+#pragma unroll
+    for (int k = 0; k < KY; ++k) {
+        const int e = tid + k * NTF;
+        const int r = e / BJ, c = e % BJ;
+        const int gi = i0 - R + r, gj = j0 + c;
+        if (e >= HR * BJ || gi < 0 || gi >= g.NX || gj >= g.NY) continue;
+        const int p = gi * g.NY + gj;
+        const bool own = gi >= i0 && gi < i0 + BI;
+        const int le = (r + R - 1) * LC + (c + HE);  // staged index of (gi, gj)
+        float hy = y_h[k];
+        const float u1 = y_u[k];
+        if (hy_up(g, gi, gj)) {
+            const float uz = axis_k(g, u1, 2);
+            const float dEz = dfwd<ORDER, int>(sE, le, LC, gi, g.NX);
+            hy = hy - uz * 0.0f + u1 * dEz;
+        }
+#pragma unroll
+        for (int face = 0; face < 2; ++face) {
+            const int q = face_q2(g, false, face, gi);
+            if (q < 0) continue;
+            const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
+            const float der = dfwd<ORDER, int>(sE, le, LC, gi, g.NX) / g.d[0];
+            const long long pi = psi_at(g, false, face, s, gi, gj, q);
+            const float phi = pp.psi_h[0][face][pi];
+            hy += f.hrhs[p] * (kc.ra1 * der + kc.rb * phi);
+            if (own) pp.psi_h[1][face][pi] = kc.re * phi - kc.rf * der;
+        }
+        sHy[e] = hy;
+        if (own) pp.Hy[1][sb0 + p] = hy;
+    }
+    // Hx on columns [j0-R, j0+BJ+R-1) of the tile's rows (y faces: H phi1)
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + k * NTF;
+        const int r = e / HC, c = e - (e / HC) * HC;
+        const int gi = i0 + r, gj = j0 - R + c;
+        if (e >= BI * HC || gi >= g.NX || gj < 0 || gj >= g.NY) continue;
+        const int p = gi * g.NY + gj;
+        const bool own = gj >= j0 && gj < j0 + BJ;
+        const int le = (r + HE) * LC + (c + R - 1);
+        float hx = x_h[k];
+        const float u1 = x_u[k];
+        if (hx_up(g, gi, gj)) {
+            const float uy = axis_k(g, u1, 1), uz = axis_k(g, u1, 2);
+            const float dEz = dfwd<ORDER, int>(sE, le, 1, gj, g.NY);
+            hx = hx - uy * dEz + uz * 0.0f;
+        }
+#pragma unroll
+        for (int face = 2; face < 4; ++face) {
+            const int q = face_q2(g, false, face, gj);
+            if (q < 0) continue;
+            const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
+            const float der = dfwd<ORDER, int>(sE, le, 1, gj, g.NY) / g.d[1];
+            const long long pi = psi_at(g, false, face, s, gi, gj, q);
+            const float phi = pp.psi_h[0][face][pi];
+            hx -= f.hrhs[p] * (kc.ra1 * der + kc.rb * phi);
+            if (own) pp.psi_h[1][face][pi] = kc.re * phi - kc.rf * der;
+        }
+        sHx[e] = hx;
+        if (own) pp.Hx[1][sb0 + p] = hx;
+    }
+    __syncthreads();
+
+    // Ez on the tile, sources, history
+    int s_beg = 0, s_end = 0;
+    if (src.count > 0) {
+        s_beg = src.block_off[tile + g.ftiles * s];
+        s_end = src.block_off[tile + g.ftiles * s + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int e = tid + k * NTF;
+        const int r = e / BJ, c = e % BJ;
+        const int gi = i0 + r, gj = j0 + c;
+        if (e >= BI * BJ || gi >= g.NX || gj >= g.NY) continue;
+        const int p = gi * g.NY + gj;
+        const float eold = sE[(r + HE) * LC + (c + HE)];
+        const int ly = (r + R) * BJ + c;  // sHy index of (gi, gj)
+        const int lx = r * HC + (c + R);  // sHx index of (gi, gj)
+        const bool up = ez_up(g, gi, gj);
+        const float u0 = t_ca[k];
+        float ez = eold;
+        if (up) {
+            const float u1 = t_cc[k];
+            const float uy = axis_k(g, u1, 1);
+            const float dHy = dbwd<ORDER, int>(sHy, ly, BJ, gi, g.NX);
+            const float dHx = dbwd<ORDER, int>(sHx, lx, 1, gj, g.NY);
+            ez = u0 * ez + u1 * dHy - uy * dHx;
+        }
+        const float cbv = t_cb[k];
+#pragma unroll
+        for (int face = 0; face < 4; ++face) {
+            const int q = face_q2(g, true, face, face < 2 ? gi : gj);
+            if (q < 0) continue;
+            const Cpml4 kc = cpml_at(f.coef_e[face], g.pml[face], q);
+            const float der = face < 2 ? dbwd<ORDER, int>(sHy, ly, BJ, gi, g.NX) / g.d[0]
+                                       : dbwd<ORDER, int>(sHx, lx, 1, gj, g.NY) / g.d[1];
+            const long long pi = psi_at(g, true, face, s, gi, gj, q);
+            const float phi = pp.psi_e[0][face][pi];
+            const float corr = cbv * (kc.ra1 * der + kc.rb * phi);
+            if (face < 2) ez += corr;
+            else ez -= corr;
+            pp.psi_e[1][face][pi] = kc.re * phi - kc.rf * der;
+        }
+        for (int q = s_beg; q < s_end; ++q) {
+            const int2 ent = src.entries[q];
+            if (ent.x == e) {
+                const float scale = src.wave[(long long)ent.y * src.nt + it] * src.dl / src.vol;
+                ez -= cbv * scale;
+            }
+        }
+        if (hist.active && gi >= hist.box.x0 && gi < hist.box.x0 + hist.box.nx && gj >= hist.box.y0 &&
+            gj < hist.box.y0 + hist.box.ny) {
+            const long long h = (long long)(gi - hist.box.x0) * hist.box.ny + (gj - hist.box.y0);
+            const long long base = (long long)s * hist.box.cells + h;
+            const int comps = hist.mode == 3 ? 3 : 1;
+            for (int ci = 0; ci < comps; ++ci) {
+                const bool is_ez = ci == comps - 1;
+                // Ex and Ey are identically zero on this path (and never updated: a = 1)
+                const float eo = is_ez ? eold : 0.0f;
+                const float en = is_ez ? ez : 0.0f;
+                const float a = (is_ez && up) ? u0 : 1.0f;
+                if (hist.save_e) store_hist(hist.e_base, ci * hist.e_cstride + base, eo, hist.e_kind);
+                if (hist.save_r) {
+                    const float rhs = cbv != 0.0f ? (en - a * eo) / cbv : 0.0f;
+                    store_hist(hist.r_base, ci * hist.r_cstride + base, rhs, hist.r_kind);
+                }
+            }
+        }
+        pp.Ez[1][sb0 + p] = ez;
+        sEn[e] = ez;
+    }
+    if (rx.count > 0) {
+        __syncthreads();
+        const int b0 = rx.off[tile + g.ftiles * s], b1 = rx.off[tile + g.ftiles * s + 1];
+        for (int q = b0 + tid; q < b1; q += NTF) {
+            const int2 ent = rx.ent[q];
+            rx.out[((long long)s * rx.nt + it) * rx.nrx + ent.y] = sEn[ent.x];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint: receivers^T, source and material terms, (E step)^T, (H step)^T
+// ---------------------------------------------------------------------------
+template <int ORDER>
+__global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
+                                                     TmReceiversAdj rx, TmAdjGrad ag, int it)
+{
+    using T = Tile<ORDER>;
+    constexpr int R = T::R, HE = T::HE, LR = T::LR, LC = T::LC, HR = T::HR, HC = T::HC;
+    constexpr int KS = cdiv(LR * LC, NTA), KA = cdiv(LR * BJ, NTA), KB = cdiv(BI * LC, NTA),
+                  KY = cdiv(HR * BJ, NTA), KX = cdiv(BI * HC, NTA), KE = cdiv(BI * BJ, NTA);
+    __shared__ float sL[LR * LC];   // lambda_Ez after receivers^T
+    __shared__ float sA[LR * BJ];   // (Hy, x) weights of (E step)^T, then (Ez, x) weights of (H step)^T
+    __shared__ float sB[BI * LC];   // (Hx, y) weights, then (Ez, y) weights
+    __shared__ float sHy[HR * BJ];  // new lambda_Hy on rows [i0-R, i0+BI+R-1)
+    __shared__ float sHx[BI * HC];  // new lambda_Hx on columns [j0-R, j0+BJ+R-1)
+    const int tile = step_tile(g.ftiles);
+    if (tile < 0) return;
+    const int s = blockIdx.y, tid = threadIdx.x;
+    const int ti = tile / g.ntj, tj = tile - ti * g.ntj;
+    const int i0 = ti * BI, j0 = tj * BJ;
+    const int lt = tile + g.ftiles * s;
+    const long long sb0 = (long long)s * g.plane;
+    const float* __restrict__ Lz = pp.Ez[0] + sb0;
+    const float* __restrict__ Lx = pp.Hx[0] + sb0;
+    const float* __restrict__ Ly = pp.Hy[0] + sb0;
+
+    // Every load that does not depend on LDS is issued before the first barrier.
+    float st[KS];
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+        const int e = tid + k * NTA;
+        const int r = e / LC, c = e - (e / LC) * LC;
+        const int gi = i0 - HE + r, gj = j0 - HE + c;
+        st[k] = (e < LR * LC && gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY) ? Lz[gi * g.NY + gj] : 0.0f;
+    }
+    float a_cc[KA];  // ccurl of the x-weight rows
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+        const int e = tid + k * NTA;
+        const int gi = i0 - HE + e / BJ, gj = j0 + e % BJ;
+        a_cc[k] = (e < LR * BJ && gi >= 0 && gi < g.NX && gj < g.NY) ? f.ccurl[gi * g.NY + gj] : 0.0f;
+    }
+    float b_cc[KB];  // ccurl of the y-weight columns
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int e = tid + k * NTA;
+        const int gi = i0 + e / LC, gj = j0 - HE + e % LC;
+        b_cc[k] = (e < BI * LC && gi < g.NX && gj >= 0 && gj < g.NY) ? f.ccurl[gi * g.NY + gj] : 0.0f;
+    }
+    float y_l[KY], y_u[KY];  // lambda_Hy and hcurl of the Hy strip
+#pragma unroll
+    for (int k = 0; k < KY; ++k) {
+        const int e = tid + k * NTA;
+        const int gi = i0 - R + e / BJ, gj = j0 + e % BJ;
+        const bool ok = e < HR * BJ && gi >= 0 && gi < g.NX && gj < g.NY;
+        y_l[k] = ok ? Ly[gi * g.NY + gj] : 0.0f;
+        y_u[k] = ok ? f.hcurl[gi * g.NY + gj] : 0.0f;
+    }
+    float x_l[KX], x_u[KX];  // lambda_Hx and hcurl of the Hx strip
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + k * NTA;
+        const int gi = i0 + e / HC, gj = j0 - R + e % HC;
+        const bool ok = e < BI * HC && gi < g.NX && gj >= 0 && gj < g.NY;
+        x_l[k] = ok ? Lx[gi * g.NY + gj] : 0.0f;
+        x_u[k] = ok ? f.hcurl[gi * g.NY + gj] : 0.0f;
+    }
+    float t_ca[KE], t_cb[KE];  // coefficients of the tile cells
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int e = tid + k * NTA;
+        const int gi = i0 + e / BJ, gj = j0 + e % BJ;
+        const bool ok = e < BI * BJ && gi < g.NX && gj < g.NY;
+        t_ca[k] = ok ? f.ca[gi * g.NY + gj] : 0.0f;
+        t_cb[k] = ok ? f.cb[gi * g.NY + gj] : 0.0f;
+    }
+    // history of a saved step (tile cells inside the history box)
+    const MatGrad& mg = ag.mg;
+    const Box& hb = mg.box;
+    float m_e[KE], m_s[KE], m_sig[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int e = tid + k * NTA;
+        const int gi = i0 + e / BJ, gj = j0 + e % BJ;
+        const bool ok = ag.terms && e < BI * BJ && gi < g.NX && gj < g.NY && gi >= hb.x0 && gi < hb.x0 + hb.nx &&
+                        gj >= hb.y0 && gj < hb.y0 + hb.ny;
+        m_e[k] = m_s[k] = m_sig[k] = 0.0f;
+        if (ok) {
+            const long long base = (long long)s * hb.cells + (long long)(gi - hb.x0) * hb.ny + (gj - hb.y0);
+            m_e[k] = load_hist(mg.eframe, ag.ez_comp * mg.e_cstride + base, mg.kind);
+            m_s[k] = load_hist(mg.sframe, ag.ez_comp * mg.s_cstride + base, mg.kind);
+            m_sig[k] = mg.sig[(long long)gi * g.NY * 2 + (long long)gj * 2];  // full-layout material index
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k)
+        if (tid + k * NTA < LR * LC) sL[tid + k * NTA] = st[k];
+    __syncthreads();
+    if (rx.count > 0) {
+        const int b0 = rx.off[lt], b1 = rx.off[lt + 1];
+        for (int q = b0 + tid; q < b1; q += NTA) {
+            const int4 grp = rx.grp[q];
+            float acc = sL[grp.x];
+            for (int m = grp.y; m < grp.y + grp.z; ++m)
+                acc += rx.dgrad[((long long)s * rx.nt + it) * rx.nrx + rx.member[m]];
+            sL[grp.x] = acc;
+        }
+        __syncthreads();
+    }
+    // source^T: one value per (step, shot, source), summed over shots later
+    if (ag.src_vals && src.count > 0) {
+        const int q0 = src.block_off[lt], q1 = src.block_off[lt + 1];
+        for (int q = q0 + tid; q < q1; q += NTA) {
+            const int2 ent = src.entries[q];
+            const int r = ent.x / BJ, c = ent.x % BJ;
+            const int p = (i0 + r) * g.NY + (j0 + c);
+            ag.src_vals[((long long)it * g.nshot + s) * ag.nsrc + ent.y] =
+                -f.cb[p] * ag.geo * sL[(r + HE) * LC + (c + HE)];
+        }
+    }
+    // per-shot material-gradient terms of a saved step
+    if (ag.terms) {
+        const long long sstride = (long long)g.nshot * hb.cells;
+        const float wgt = (float)mg.weight;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int e = tid + k * NTA;
+            const int r = e / BJ, c = e % BJ;
+            const int gi = i0 + r, gj = j0 + c;
+            if (e >= BI * BJ || gi >= g.NX || gj >= g.NY) continue;
+            if (gi < hb.x0 || gi >= hb.x0 + hb.nx || gj < hb.y0 || gj >= hb.y0 + hb.ny) continue;
+            const long long base = (long long)s * hb.cells + (long long)(gi - hb.x0) * hb.ny + (gj - hb.y0);
+            const float cav = t_ca[k], cbv = t_cb[k];
+            const bool up = ez_up(g, gi, gj);
+            const float e_old = m_e[k];
+            const float sec = m_s[k];
+            float lam = sL[(r + HE) * LC + (c + HE)];
+            float rhs = sec;
+            if (mg.e_only) {
+                const float a = up ? cav : 1.0f;
+                rhs = cbv != 0.0f ? (sec - a * e_old) / cbv : 0.0f;
+            }
+            if (e_old == 0.0f && rhs == 0.0f) lam = 0.0f;
+            const float gca = up ? lam * e_old * wgt : 0.0f;
+            const float gcb = lam * rhs * wgt;
+            const bool active = m_sig[k] <= kPecSigma;
+            if (mg.want_eps) {
+                float t = 0.0f;
+                if (active) {
+                    const float dca = kEps0 * (1.0f - cav) * cbv / mg.dt;
+                    const float dcb = -kEps0 * cbv * cbv / mg.dt;
+                    t = gca * dca + gcb * dcb;
+                }
+                ag.terms[base] = t;
+            }
+            if (mg.want_sig) {
+                float t = 0.0f;
+                if (active) {
+                    const float dca = -0.5f * (1.0f + cav) * cbv;
+                    const float dcb = -0.5f * cbv * cbv;
+                    t = gca * dca + gcb * dcb;
+                }
+                ag.terms[sstride + base] = t;
+            }
+        }
+    }
+
+    // (E step)^T weights on rows [i0-HE, i0+BI+HE); owners advance lambda psi_E of the x faces
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+        const int e = tid + k * NTA;
+        const int r = e / BJ, c = e % BJ;
+        const int gi = i0 - HE + r, gj = j0 + c;
+        if (e >= LR * BJ || gi < 0 || gi >= g.NX || gj >= g.NY) continue;
+        const int p = gi * g.NY + gj;
+        const float lz = sL[r * LC + (c + HE)];
+        const bool own = gi >= i0 && gi < i0 + BI;
+        float acc = 0.0f;
+#pragma unroll
+        for (int face = 0; face < 2; ++face) {
+            const int q = face_q2(g, true, face, gi);
+            if (q < 0) continue;
+            const Cpml4 kc = cpml_at(f.coef_e[face], g.pml[face], q);
+            const long long pi = psi_at(g, true, face, s, gi, gj, q);
+            const float old = pp.psi_e[0][face][pi];
+            const float u = f.cb[p];
+            const float inv = 1.0f / g.d[0];
+            acc += (1.0f * u * kc.ra1 * lz - kc.rf * old) * inv;
+            if (own) pp.psi_e[1][face][pi] = 1.0f * u * kc.rb * lz + kc.re * old;
+        }
+        if (ez_up(g, gi, gj)) acc += axis_k(g, a_cc[k], 0) * lz;
+        sA[e] = acc;
+    }
+    // ... on columns [j0-HE, j0+BJ+HE); owners advance lambda psi_E of the y faces
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        const int e = tid + k * NTA;
+        const int r = e / LC, c = e - (e / LC) * LC;
+        const int gi = i0 + r, gj = j0 - HE + c;
+        if (e >= BI * LC || gi >= g.NX || gj < 0 || gj >= g.NY) continue;
+        const int p = gi * g.NY + gj;
+        const float lz = sL[(r + HE) * LC + c];
+        const bool own = gj >= j0 && gj < j0 + BJ;
+        float acc = 0.0f;
+#pragma unroll
+        for (int face = 2; face < 4; ++face) {
+            const int q = face_q2(g, true, face, gj);
+            if (q < 0) continue;
+            const Cpml4 kc = cpml_at(f.coef_e[face], g.pml[face], q);
+            const long long pi = psi_at(g, true, face, s, gi, gj, q);
+            const float old = pp.psi_e[0][face][pi];
+            const float u = f.cb[p];
+            const float inv = 1.0f / g.d[1];
+            acc += (-1.0f * u * kc.ra1 * lz - kc.rf * old) * inv;
+            if (own) pp.psi_e[1][face][pi] = -1.0f * u * kc.rb * lz + kc.re * old;
+        }
+        if (ez_up(g, gi, gj)) acc += -axis_k(g, b_cc[k], 1) * lz;
+        sB[e] = acc;
+    }
+    __syncthreads();
+
+    // lambda_Hy on rows [i0-R, i0+BI+R-1), lambda_Hx on columns [j0-R, j0+BJ+R-1)
+#pragma unroll
+    for (int k = 0; k < KY; ++k) {
+        const int e = tid + k * NTA;
+        const int r = e / BJ, c = e % BJ;
+        const int gi = i0 - R + r, gj = j0 + c;
+        if (e >= HR * BJ || gi < 0 || gi >= g.NX || gj >= g.NY) continue;
+        const int base = (r + R - 1) * BJ + c;  // sA row of gi is gi - (i0 - HE)
+        auto wx = [&](int off) { return sA[base + off * BJ]; };
+        // the (Hy, x) channel, then the identically zero (Hy, z) channel
+        const float v = (y_l[k] + gather_bwd_T<ORDER>(wx, gi, g.NX)) + 0.0f;
+        sHy[e] = v;
+        if (gi >= i0 && gi < i0 + BI) pp.Hy[1][sb0 + gi * g.NY + gj] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + k * NTA;
+        const int r = e / HC, c = e - (e / HC) * HC;
+        const int gi = i0 + r, gj = j0 - R + c;
+        if (e >= BI * HC || gi >= g.NX || gj < 0 || gj >= g.NY) continue;
+        const int base = r * LC + (c + R - 1);  // sB column of gj is gj - (j0 - HE)
+        auto wy = [&](int off) { return sB[base + off]; };
+        const float v = (x_l[k] + gather_bwd_T<ORDER>(wy, gj, g.NY)) + 0.0f;
+        sHx[e] = v;
+        if (gj >= j0 && gj < j0 + BJ) pp.Hx[1][sb0 + gi * g.NY + gj] = v;
+    }
+    __syncthreads();
+
+    // (H step)^T weights into sA (rows [i0-R, ...)) and sB (columns [j0-R, ...)); owners advance lambda psi_H
+#pragma unroll
+    for (int k = 0; k < KY; ++k) {
+        const int e = tid + k * NTA;
+        const int r = e / BJ, c = e % BJ;
+        const int gi = i0 - R + r, gj = j0 + c;
+        if (e >= HR * BJ || gi < 0 || gi >= g.NX || gj >= g.NY) continue;
+        const int p = gi * g.NY + gj;
+        const float ly = sHy[e];
+        const bool own = gi >= i0 && gi < i0 + BI;
+        float acc = 0.0f;
+#pragma unroll
+        for (int face = 0; face < 2; ++face) {
+            const int q = face_q2(g, false, face, gi);
+            if (q < 0) continue;
+            const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
+            const long long pi = psi_at(g, false, face, s, gi, gj, q);
+            const float old = pp.psi_h[0][face][pi];
+            const float u = f.hrhs[p];
+            const float inv = 1.0f / g.d[0];
+            acc += (1.0f * u * kc.ra1 * ly - kc.rf * old) * inv;
+            if (own) pp.psi_h[1][face][pi] = 1.0f * u * kc.rb * ly + kc.re * old;
+        }
+        if (hy_up(g, gi, gj)) acc += axis_k(g, y_u[k], 0) * ly;
+        sA[e] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        const int e = tid + k * NTA;
+        const int r = e / HC, c = e - (e / HC) * HC;
+        const int gi = i0 + r, gj = j0 - R + c;
+        if (e >= BI * HC || gi >= g.NX || gj < 0 || gj >= g.NY) continue;
+        const int p = gi * g.NY + gj;
+        const float lx = sHx[e];
+        const bool own = gj >= j0 && gj < j0 + BJ;
+        float acc = 0.0f;
+#pragma unroll
+        for (int face = 2; face < 4; ++face) {
+            const int q = face_q2(g, false, face, gj);
+            if (q < 0) continue;
+            const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
+            const long long pi = psi_at(g, false, face, s, gi, gj, q);
+            const float old = pp.psi_h[0][face][pi];
+            const float u = f.hrhs[p];
+            const float inv = 1.0f / g.d[1];
+            acc += (-1.0f * u * kc.ra1 * lx - kc.rf * old) * inv;
+            if (own) pp.psi_h[1][face][pi] = -1.0f * u * kc.rb * lx + kc.re * old;
+        }
+        if (hx_up(g, gi, gj)) acc += -axis_k(g, x_u[k], 1) * lx;
+        sB[e] = acc;
+    }
+    __syncthreads();
+
+    // lambda_Ez = (ca) lambda_Ez + D+^T (x weights) + D+^T (y weights)
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int e = tid + k * NTA;
+        const int r = e / BJ, c = e % BJ;
+        const int gi = i0 + r, gj = j0 + c;
+        if (e >= BI * BJ || gi >= g.NX || gj >= g.NY) continue;
+        float acc = sL[(r + HE) * LC + (c + HE)];
+        if (ez_up(g, gi, gj)) acc = t_ca[k] * acc;
+        const int bx = (r + R) * BJ + c;  // sA row of gi is gi - (i0 - R)
+        const int by = r * HC + (c + R);  // sB column of gj is gj - (j0 - R)
+        auto vx = [&](int off) { return sA[bx + off * BJ]; };
+        auto vy = [&](int off) { return sB[by + off]; };
+        acc += gather_fwd_T<ORDER>(vx, gi, g.NX);
+        acc += gather_fwd_T<ORDER>(vy, gj, g.NY);
+        pp.Ez[1][sb0 + gi * g.NY + gj] = acc;
+    }
+}
+
+// Material gradient of one saved step: per-shot terms summed in shot order.
+template <typename ACC>
+__global__ __launch_bounds__(256) void k_tm_grad_reduce(Tm2dGrid g, Box b, const float* __restrict__ terms,
+                                                        int want_eps, int want_sig, ACC* __restrict__ geps,
+                                                        ACC* __restrict__ gsig)
+{
+    const long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= b.cells) return;
+    const int i = (int)(h / b.ny) + b.x0, j = (int)(h % b.ny) + b.y0;
+    const long long out = (long long)i * (g.NY - 1) + j;
+    const long long sstride = (long long)g.nshot * b.cells;
+    if (want_eps) {
+        ACC acc = 0;
+        for (int s = 0; s < g.nshot; ++s) acc += (ACC)terms[(long long)s * b.cells + h];
+        geps[out] += acc;
+    }
+    if (want_sig) {
+        ACC acc = 0;
+        for (int s = 0; s < g.nshot; ++s) acc += (ACC)terms[sstride + (long long)s * b.cells + h];
+        gsig[out] += acc;
+    }
+}
+
+// grad_source[k, it] += per-shot values, shots in order
+__global__ __launch_bounds__(256) void k_tm_src_reduce(const float* __restrict__ vals, float* __restrict__ gsrc,
+                                                       int nshot, int nsrc, int nt)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nsrc * nt) return;
+    const int k = (int)(idx / nt), it = (int)(idx - (long long)k * nt);
+    float acc = gsrc[idx];
+    for (int s = 0; s < nshot; ++s) acc += vals[((long long)it * nshot + s) * nsrc + k];
+    gsrc[idx] = acc;
+}
+
+// --- launchers -----------------------------------------------------------------
+
+static dim3 step_grid(const Tm2dGrid& g) { return dim3((unsigned)((g.ftiles + 7) / 8 * 8), (unsigned)g.nshot); }
+
+void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                        const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
+                        hipStream_t st)
+{
+    const dim3 grid = step_grid(g);
+    if (order == 8) hipLaunchKernelGGL(k_tm_step_fwd<8>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it);
+    else if (order == 4) hipLaunchKernelGGL(k_tm_step_fwd<4>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it);
+    else hipLaunchKernelGGL(k_tm_step_fwd<2>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it);
+}
+
+void launch_tm_step_adj(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                        const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it,
+                        hipStream_t st)
+{
+    const dim3 grid = step_grid(g);
+    if (order == 8) hipLaunchKernelGGL(k_tm_step_adj<8>, grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+    else if (order == 4) hipLaunchKernelGGL(k_tm_step_adj<4>, grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+    else hipLaunchKernelGGL(k_tm_step_adj<2>, grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+}
+
+void launch_tm_grad_reduce(const Tm2dGrid& g, const Box& b, const float* terms, int want_eps, int want_sig,
+                           void* geps, void* gsig, bool fp64, hipStream_t st)
+{
+    if (b.cells <= 0) return;
+    const unsigned blocks = (unsigned)((b.cells + 255) / 256);
+    if (fp64)
+        hipLaunchKernelGGL(k_tm_grad_reduce<double>, dim3(blocks), dim3(256), 0, st, g, b, terms, want_eps,
+                           want_sig, (double*)geps, (double*)gsig);
+    else
+        hipLaunchKernelGGL(k_tm_grad_reduce<float>, dim3(blocks), dim3(256), 0, st, g, b, terms, want_eps,
+                           want_sig, (float*)geps, (float*)gsig);
+}
+
+void launch_tm_src_reduce(const float* vals, float* gsrc, int nshot, int nsrc, int nt, hipStream_t st)
+{
+    const long long n = (long long)nsrc * nt;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_tm_src_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vals, gsrc, nshot,
+                       nsrc, nt);
+}
+
+}  // namespace dgx
