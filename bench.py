@@ -36,6 +36,8 @@ def _parse():
     ap.add_argument("--storage", default="float32", choices=("float32", "bfloat16", "float16", "int8"))
     ap.add_argument("--rhs", default="auto", choices=("auto", "stored", "reconstructed"))
     ap.add_argument("--accumulator", default="float64", choices=("float32", "float64"))
+    ap.add_argument("--shots-per-call", type=int, default=0,
+                    help="shots per compute() call (0: all that fit in free HBM)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
@@ -90,53 +92,54 @@ def _psi_elems(w):
 
 
 def _history_terms(w, args):
+    """(components, bytes per stored element, frames per saved step) of the history."""
     comps = 3 if w.mode == 3 else 1
-    hist_b = {"float32": 4, "bfloat16": 2, "float16": 2, "int8": 1}[args.storage]
+    two_d = len(w.physical_shape) == 2
+    tile = 64  # INT8 scale tile: 8x8x1 in 2D, 4x4x4 in 3D
+    hist_b = {"float32": 4.0, "bfloat16": 2.0, "float16": 2.0, "int8": 1.0 + 4.0 / tile}[args.storage]
     e_only = args.rhs == "reconstructed" or (args.rhs == "auto" and args.storage == "float32"
                                              and w.sampling_interval == 1)
-    frames = 1 if e_only else 2  # E^n (+ R^n) per saved step; E-only frames are each read once
-    return comps, hist_b, frames
+    return comps, hist_b, (1 if e_only else 2)
 
 
-def _direction_bytes(w, args, adjoint: bool) -> float:
-    """Compulsory HBM bytes of one time step of one direction, all shots:
-    every field read and written once, each coefficient plane read once (it is
-    shared by all shots), the active CPML auxiliaries read and written once,
-    the history frames written (forward) or read (adjoint), and the gradient
-    accumulators read and written once per saved step."""
-    shots = w.source_location.shape[0]
-    cells = w.extended_cells()
+def model_bytes_per_cell(w, args):
+    """Algorithmic HBM bytes per cell-update (forward, adjoint) of a one-sweep
+    leapfrog step: fields read and written plus ca and cb (2D TM 32 B, 3D 56 B),
+    8 B per active CPML auxiliary element, the history written (forward) and
+    read (adjoint, E^n and E^(n+1) or R^n), and the two gradient accumulators
+    read and written once per saved step."""
     two_d = len(w.physical_shape) == 2
-    pe, ph = (x * shots for x in _psi_elems(w))
+    core = 32.0 if two_d else 56.0
+    pe, ph = _psi_elems(w)
+    cpml = 8.0 * (pe + ph) / w.extended_cells()
     comps, hist_b, frames = _history_terms(w, args)
-    fields = 3 if two_d else 6
-    planes = 4 if two_d else 5  # ca, ccurl, cb, hcurl (+ hrhs in 3D where every cell may be CPML)
-    total = shots * cells * fields * 8 + cells * planes * 4 + 8 * (pe + ph)
     S = w.sampling_interval
-    if not adjoint:
-        total += shots * cells * comps * hist_b * frames / S
-    else:
-        read_comps = 1 if two_d else comps  # the 2D TM gradient reads only the Ez component
-        acc = 8 if args.accumulator == "float64" else 4
-        total += (shots * cells * read_comps * hist_b * frames + cells * (4 + 2 * 2 * acc)) / S
-    return total
+    acc = 8.0 if args.accumulator == "float64" else 4.0
+    fwd = core + cpml + comps * hist_b * frames / S
+    adj = core + cpml + 2 * comps * hist_b / S + 2 * 2 * acc / S
+    return fwd, adj
 
 
 def kernel_bytes(kernel: str, w, args) -> float:
-    """Algorithmic HBM bytes of ONE launch of a kernel (one time step, all shots):
-    the compulsory loads/stores of that kernel's function, each array element
-    counted once per launch."""
-    shots = w.source_location.shape[0]
-    cells = w.extended_cells() * shots
+    """Algorithmic bytes of ONE launch of a kernel (one time step, all shots of
+    one call). The fused 2D TM kernels carry the per-cell model above times
+    the cells they advance; the gradient accumulators belong to the reduction
+    kernel while it is separate. The split kernels carry their own share of
+    the compulsory loads/stores."""
+    shots = args.shots_per_call or w.source_location.shape[0]
+    ext = w.extended_cells()
+    cells = ext * shots
     two_d = len(w.physical_shape) == 2
     pe, ph = (x * shots for x in _psi_elems(w))
     comps, hist_b, frames = _history_terms(w, args)
+    fwd, adj = model_bytes_per_cell(w, args)
+    acc_b = 2 * 2 * (8.0 if args.accumulator == "float64" else 4.0) / w.sampling_interval
     if kernel == "tm_step_fwd":
-        return _direction_bytes(w, args, adjoint=False)
+        return fwd * cells
     if kernel == "tm_step_adj":
-        # the saved-step share of the material gradient is in tm_grad_reduce
-        acc = 8 if args.accumulator == "float64" else 4
-        return _direction_bytes(w, args, adjoint=True) - w.extended_cells() * (4 + 2 * 2 * acc) / w.sampling_interval
+        return (adj - acc_b) * cells
+    if kernel == "tm_grad_reduce":
+        return acc_b * w.sampling_interval * ext  # one launch per saved step, accumulators of all shots
     if kernel == "fwd_h":
         core = 24 if two_d else 40      # E read, H read+write, ch_curl
         return cells * core + 8 * ph
@@ -151,15 +154,27 @@ def kernel_bytes(kernel: str, w, args) -> float:
         return cells * core + 4 * ph
     if kernel == "material_gradient":
         acc = 8 if args.accumulator == "float64" else 4
-        return cells * (comps * 4 + 2 * comps * hist_b + 12 + 2 * 2 * acc)
+        return cells * (2 * comps * hist_b + 12) + ext * 2 * 2 * acc
     if kernel in ("adj_pml_e", "adj_pml_h"):
         return 12 * (pe if kernel == "adj_pml_e" else ph)
     return 0.0
 
 
-def step_bytes(w, args) -> float:
-    """Algorithmic bytes of one forward + one adjoint time step (all shots)."""
-    return _direction_bytes(w, args, adjoint=False) + _direction_bytes(w, args, adjoint=True)
+def _shots_per_call(w, args, free_bytes: int) -> int:
+    """Shots per compute() call: all of them when the whole solve fits in 85 %
+    of free HBM, else as many as fit (a history larger than HBM runs shot by shot)."""
+    from deepgpr_amd.solver.memory import estimate_compute_memory
+
+    shape = [n + 2 * w.pml for n in w.physical_shape] + ([1] if len(w.physical_shape) == 2 else [])
+    comps, hist_b, frames = _history_terms(w, args)
+    one = estimate_compute_memory(*shape, w.nt, 1, [w.pml] * 6, mode=w.mode, sampling_interval=w.sampling_interval,
+                                  storage="float32" if args.storage == "int8" else args.storage,
+                                  compression="int8" if args.storage == "int8" else "none",
+                                  reconstruct_rhs=frames == 1, fp64_accumulators=args.accumulator == "float64",
+                                  nrx=w.receiver_location.shape[1])["total"]
+    shots = w.source_location.shape[0]
+    fit = max(1, int(0.85 * free_bytes // max(one, 1)))
+    return min(shots, fit)
 
 
 # ----------------------------------------------------------------------------- CPU baseline
@@ -168,40 +183,16 @@ def step_bytes(w, args) -> float:
 def _cpu_baseline_child(spec: str) -> None:
     """Runs in a subprocess: the reference CPU backend (or the oracle) on a
     bounded sample of the workload; prints one JSON line."""
-    import numpy as np
-    import torch
-
+    from deepgpr_amd import synthetic
     from oracle import harness as H
 
     args = json.loads(spec)
-    from deepgpr_amd import synthetic
-    from deepgpr_amd.preprocessing.model_setup import initialization, shift_locations
-    from deepgpr_amd.solver.pml import build_pml_coeffs
-
     w = synthetic.WORKLOADS[args["workload"]]() if args["workload"] != "c1" else synthetic.c1_small_2d()
-    prep = initialization("cpu", w.eps_r, w.sigma, None, w.source_amplitudes, w.source_location[:1],
-                          w.receiver_location[:1], w.dx, w.dt, w.pml, w.fdtd_order)
-    coeffs = build_pml_coeffs(prep.er, prep.mr, w.dt, w.dx, prep.nx, prep.ny, prep.nz, prep.pmlthick, "cpu",
-                              torch.float32)[6:]
-    src = shift_locations(w.source_location[:1], prep.pmlthick, "cpu").numpy()
-    rx = shift_locations(w.receiver_location[:1], prep.pmlthick, "cpu").numpy()
-    NX, NY, NZ = prep.nx + 1, prep.ny + 1, prep.nz + 1
     kind = "reference" if H.reference_available() else "port"
     which = "ref" if kind == "reference" else "oracle"
 
     def problem(nt):
-        p = H.make_problem(NX, NY, NZ, order=w.fdtd_order, nt=nt, nshot=1, nsrc=1, nrx=rx.shape[1], seed=1,
-                           random_state=False, pml=[int(v) for v in prep.pmlthick], d=(w.dx, w.dx, w.dx))
-        p.dt = w.dt
-        p.eps, p.sig, p.mu = (t.detach().numpy().astype(np.float32) for t in (prep.ere, prep.see, prep.mr))
-        p.coef_e = [coeffs[2 * f].numpy() for f in range(6)]
-        p.coef_h = [coeffs[2 * f + 1].numpy() for f in range(6)]
-        p.src_loc, p.rx_loc = src.astype(np.int32), rx.astype(np.int32)
-        p.wave = w.source_amplitudes[0, :nt, 0].numpy()[None].astype(np.float32)
-        p.mode, p.S = w.mode, w.sampling_interval
-        p.storage_type = H.RHS_FROM_E if w.sampling_interval == 1 else 0
-        p.data_grad = np.random.default_rng(0).standard_normal((1, nt, rx.shape[1])).astype(np.float32)
-        return p
+        return H.workload_problem(w, 1, nt, H.RHS_FROM_E if w.sampling_interval == 1 else 0)
 
     calib = min(40, w.nt)
     t0 = time.perf_counter()
@@ -212,7 +203,7 @@ def _cpu_baseline_child(spec: str) -> None:
     t0 = time.perf_counter()
     H.run_cpu(p, which)
     elapsed = time.perf_counter() - t0
-    cells = (NX - 1) * (NY - 1) * (NZ - 1)
+    cells = w.extended_cells()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     print(json.dumps({"value": 2 * cells * nt / elapsed / 1e9, "unit": "Gcell-updates/s", "cores": threads,
                       "kind": kind, "sample": f"{w.name}: shot 1 of {w.source_location.shape[0]}, nt={nt} of "
@@ -267,16 +258,24 @@ def main():
     src_loc = w.source_location.to(dev)
     rx_loc = w.receiver_location.to(dev)
 
+    nshot = w.source_location.shape[0]
+    if not args.shots_per_call:
+        args.shots_per_call = _shots_per_call(w, args, torch.cuda.mem_get_info(dev)[0])
+    chunk = args.shots_per_call
+
     def step():
         e = eps.detach().requires_grad_(True)
         s = sig.detach().requires_grad_(True)
-        out = dg.compute(dev, w.dx, w.dt, src_amp, src_loc, rx_loc, eps_r=e, sigma=s, **kw)
-        data = out[-1]
-        (data * data).sum().backward()
+        for c0 in range(0, nshot, chunk):
+            out = dg.compute(dev, w.dx, w.dt, src_amp, src_loc[c0:c0 + chunk], rx_loc[c0:c0 + chunk], eps_r=e,
+                             sigma=s, **kw)
+            data = out[-1]
+            (data * data).sum().backward()
+            del out, data
         return e.grad, s.grad
 
     print(f"[bench] {w.name} storage={args.storage} rhs={args.rhs} acc={args.accumulator} "
-          f"warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+          f"warmup={args.warmup} steps={args.steps} shots/call={chunk}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize(dev)
@@ -316,18 +315,20 @@ def main():
             roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                         "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_launch)}
-    pair_bytes = step_bytes(w, args) / (w.extended_cells() * w.source_location.shape[0])
+    fwd_b, adj_b = model_bytes_per_cell(w, args)
+    pair_bytes = fwd_b + adj_b
     cpu = None if args.no_cpu_baseline else _cpu_baseline(args)
     line = {
         "metric": "Gcell-updates/s fwd+adjoint (% HBM roofline); serve p50/p99 ms vs plain solve",
         "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": w.name, "shots": int(w.source_location.shape[0]), "nt": w.nt,
+        "config": {"workload": w.name, "shots": int(w.source_location.shape[0]), "shots_per_call": chunk, "nt": w.nt,
                    "fdtd_order": w.fdtd_order, "history": args.storage, "rhs_history": args.rhs,
                    "gradient_accumulator": args.accumulator, "sampling_interval": w.sampling_interval},
         "roofline": roofline,
-        "step_roofline": {"algorithmic_bytes_per_cell_pair": round(pair_bytes, 1),
+        "step_roofline": {"bytes_per_cell_update": [round(fwd_b, 2), round(adj_b, 2)],
+                          "roofline_gcell_per_s": round(2 * HBM_PEAK_GBS / pair_bytes, 1),
                           "frac": round(value * 1e9 / 2 * pair_bytes / (HBM_PEAK_GBS * 1e9), 4)},
         "kernels_ms_per_launch": {n: round(t / c, 4) for n, (t, c) in kernels.items()},
         "cpu_baseline": cpu,

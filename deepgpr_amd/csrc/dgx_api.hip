@@ -27,19 +27,32 @@ void set_error(const char* fmt, ...)
 }
 
 // Device allocation released at scope exit (after the stream has drained).
+// Stream of the solve running on this thread. Scratch is allocated and freed
+// stream-ordered on it, so concurrent solves on different streams never wait
+// for each other (hipFree would synchronise the whole device).
+thread_local hipStream_t g_call_stream = nullptr;
+
+struct CallStream {
+    hipStream_t prev;
+    explicit CallStream(hipStream_t s) : prev(g_call_stream) { g_call_stream = s; }
+    ~CallStream() { g_call_stream = prev; }
+};
+
 struct DevBuf {
     void* p = nullptr;
+    hipStream_t st = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf()
     {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, st);
     }
     bool alloc(size_t bytes)
     {
         if (bytes == 0) bytes = 16;
-        return hipMalloc(&p, bytes) == hipSuccess;
+        st = g_call_stream;
+        return hipMallocAsync(&p, bytes, st) == hipSuccess;
     }
 };
 
@@ -570,6 +583,7 @@ bool upload(DevBuf& d, const std::vector<T>& v, hipStream_t st)
 
 int run_forward(const dgx_solve* P, hipStream_t stream)
 {
+    CallStream call(stream);
     Plan pl;
     if (make_plan(P, false, pl)) return -1;
     const Grid& g = pl.g;
@@ -823,9 +837,13 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
             TIMED(tmr, kKSaveFrame, launch_save_frame(g, a.Ec, hw, stream));
         }
     }
+    // Asynchronous like the reference: launch errors are reported here, the
+    // caller's stream orders everything after (scratch is freed on it too).
     DGX_TRY(hipGetLastError(), "forward: kernel launch");
-    DGX_TRY(hipStreamSynchronize(stream), "forward: stream synchronize");
-    tmr.collect();
+    if (tmr.on) {
+        DGX_TRY(hipStreamSynchronize(stream), "forward: stream synchronize");
+        tmr.collect();
+    }
     return 0;
 }
 
@@ -860,6 +878,7 @@ MatGrad mat_step(const Plan& pl, const dgx_solve* P, MatGrad mg, int it, int nt,
 
 int run_backward(const dgx_solve* P, hipStream_t stream)
 {
+    CallStream call(stream);
     Plan pl;
     if (make_plan(P, true, pl)) return -1;
     const Grid& g = pl.g;
@@ -1035,7 +1054,9 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
         tgrad.src_vals = (float*)d_svals.p;
         tgrad.nsrc = P->nsource;
     }
-    if (fused && want_mat && !d_terms.alloc((size_t)2 * g.nshot * pl.box.cells * sizeof(float))) {
+    // two buffers: a launch writes one while it sums the other (the previous saved step)
+    const long long terms_len = 2 * (long long)g.nshot * pl.box.cells;
+    if (fused && want_mat && !d_terms.alloc((size_t)2 * terms_len * sizeof(float))) {
         set_error("backward: out of device memory for per-shot gradient terms");
         return -1;
     }
@@ -1145,20 +1166,22 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     float* lam_src = tm ? tms.f.Ez : (want_src ? P->E[P->source_component] : nullptr);
     const float* cb_src = tm ? tms.f.cb : co[2];
 
+    tgrad.geps = geps;
+    tgrad.gsig = gsig;
+    tgrad.mg = mg;
+    int terms_buf = 0;
     for (int it = nt - 1; it >= 0; --it) {
         if (fused) {
+            tgrad.pending = tgrad.terms;
             tgrad.terms = nullptr;
             if (want_mat && it % S == 0) {
                 tgrad.mg = mat_step(pl, P, mg, it, nt, S);
-                tgrad.terms = (float*)d_terms.p;
+                tgrad.terms = (float*)d_terms.p + terms_buf * terms_len;
+                terms_buf ^= 1;
             }
             TIMED(tmr, kKTmStepAdj,
-                  launch_tm_step_adj(pl.order, tms.g, tms.f, tms.pp, tsrc, tax, tgrad, it, stream));
+                  launch_tm_step_adj(pl.order, tms.g, tms.f, tms.pp, tsrc, tax, tgrad, it, fp64, stream));
             tms.flip();
-            if (tgrad.terms)
-                TIMED(tmr, kKTmGradReduce,
-                      launch_tm_grad_reduce(tms.g, pl.box, tgrad.terms, mg.want_eps, mg.want_sig, geps, gsig, fp64,
-                                            stream));
             continue;
         }
         if (adj_rx)
@@ -1191,6 +1214,9 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
         set_error("backward: copy back of the 2D TM state");
         return -1;
     }
+    if (fused && tgrad.terms)  // the last saved step has no later launch to sum it
+        TIMED(tmr, kKTmGradReduce,
+              launch_tm_grad_reduce(tms.g, pl.box, tgrad.terms, mg.want_eps, mg.want_sig, geps, gsig, fp64, stream));
     if (fused && want_src)
         TIMED(tmr, kKTmSrcReduce,
               launch_tm_src_reduce(tgrad.src_vals, P->grad_source, g.nshot, P->nsource, nt, stream));
@@ -1199,8 +1225,10 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
         if (P->sigma_requires_grad == 1) launch_add_f64((const double*)gsig, P->grad_sigma, ngrad, stream);
     }
     DGX_TRY(hipGetLastError(), "backward: kernel launch");
-    DGX_TRY(hipStreamSynchronize(stream), "backward: stream synchronize");
-    tmr.collect();
+    if (tmr.on) {
+        DGX_TRY(hipStreamSynchronize(stream), "backward: stream synchronize");
+        tmr.collect();
+    }
     return 0;
 }
 
@@ -1409,7 +1437,9 @@ DGX_API void forward(
     s.storage_type = storage_type;
     s.save_model_history = save_model_history;
     s.use_async_offload = use_async_offload;
-    (void)dgx_forward(&s, nullptr);
+    // the reference export set returns with the null stream drained
+    if (dgx_forward(&s, nullptr) == 0 && hipStreamSynchronize(nullptr) != hipSuccess)
+        set_error("forward: stream synchronize");
 }
 
 DGX_API void backward(
@@ -1484,7 +1514,9 @@ DGX_API void backward(
     s.fwi_mode = fwi_mode;
     s.storage_type = storage_type;
     s.use_async_offload = use_async_offload;
-    (void)dgx_backward(&s, nullptr);
+    // the reference export set returns with the null stream drained
+    if (dgx_backward(&s, nullptr) == 0 && hipStreamSynchronize(nullptr) != hipSuccess)
+        set_error("backward: stream synchronize");
 }
 
 }  // extern "C"

@@ -161,8 +161,11 @@ struct TmReceiversAdj {
 // Per-shot gradient terms written by the fused adjoint and summed over shots
 // in order afterwards (keeps the FP32 summation order of the unfused path).
 struct TmAdjGrad {
-    float* terms;     // [eps | sig] x (nshot, box cells); null when no material gradient this step
-    float* src_vals;  // (nt, nshot, nsrc); null without a source gradient
+    float* terms;          // [eps | sig] x (nshot, box cells) of this step; null when not a saved step
+    const float* pending;  // terms of the previous saved step, summed over shots by this launch; or null
+    void* geps;            // accumulators (float or double, as the kernel's ACC)
+    void* gsig;
+    float* src_vals;       // (nt, nshot, nsrc); null without a source gradient
     int nsrc;
     float geo;
     int ez_comp;
@@ -173,7 +176,7 @@ void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const
                         const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
                         hipStream_t st);
 void launch_tm_step_adj(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
-                        const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it,
+                        const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it, bool fp64,
                         hipStream_t st);
 void launch_tm_grad_reduce(const Tm2dGrid& g, const Box& b, const float* terms, int want_eps, int want_sig,
                            void* geps, void* gsig, bool fp64, hipStream_t st);

@@ -309,7 +309,29 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
 // ---------------------------------------------------------------------------
 // Adjoint: receivers^T, source and material terms, (E step)^T, (H step)^T
 // ---------------------------------------------------------------------------
-template <int ORDER>
+// Sum of the per-shot material terms of cell h of the history box, in shot
+// order, added to the accumulators (one saved step).
+template <typename ACC>
+__device__ __forceinline__ void reduce_terms(const Tm2dGrid& g, const Box& b, const float* __restrict__ terms,
+                                             long long h, int want_eps, int want_sig, ACC* __restrict__ geps,
+                                             ACC* __restrict__ gsig)
+{
+    const int i = (int)(h / b.ny) + b.x0, j = (int)(h % b.ny) + b.y0;
+    const long long out = (long long)i * (g.NY - 1) + j;
+    const long long sstride = (long long)g.nshot * b.cells;
+    if (want_eps) {
+        ACC acc = 0;
+        for (int s = 0; s < g.nshot; ++s) acc += (ACC)terms[(long long)s * b.cells + h];
+        geps[out] += acc;
+    }
+    if (want_sig) {
+        ACC acc = 0;
+        for (int s = 0; s < g.nshot; ++s) acc += (ACC)terms[sstride + (long long)s * b.cells + h];
+        gsig[out] += acc;
+    }
+}
+
+template <int ORDER, typename ACC>
 __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
                                                      TmReceiversAdj rx, TmAdjGrad ag, int it)
 {
@@ -626,6 +648,21 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
         acc += gather_fwd_T<ORDER>(vy, gj, g.NY);
         pp.Ez[1][sb0 + gi * g.NY + gj] = acc;
     }
+
+    // Shot sums of the previous saved step: the workgroup of shot s takes the
+    // s-th slice of its tile, so every cell is summed once.
+    if (ag.pending) {
+        const Box& b = ag.mg.box;
+        const int chunk = (BI * BJ + g.nshot - 1) / g.nshot;
+        const int e_end = min(BI * BJ, (s + 1) * chunk);
+        for (int e = s * chunk + tid; e < e_end; e += NTA) {
+            const int gi = i0 + e / BJ, gj = j0 + e % BJ;
+            if (gi >= g.NX || gj >= g.NY || gi < b.x0 || gi >= b.x0 + b.nx || gj < b.y0 || gj >= b.y0 + b.ny)
+                continue;
+            reduce_terms<ACC>(g, b, ag.pending, (long long)(gi - b.x0) * b.ny + (gj - b.y0), ag.mg.want_eps,
+                              ag.mg.want_sig, (ACC*)ag.geps, (ACC*)ag.gsig);
+        }
+    }
 }
 
 // Material gradient of one saved step: per-shot terms summed in shot order.
@@ -636,19 +673,7 @@ __global__ __launch_bounds__(256) void k_tm_grad_reduce(Tm2dGrid g, Box b, const
 {
     const long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= b.cells) return;
-    const int i = (int)(h / b.ny) + b.x0, j = (int)(h % b.ny) + b.y0;
-    const long long out = (long long)i * (g.NY - 1) + j;
-    const long long sstride = (long long)g.nshot * b.cells;
-    if (want_eps) {
-        ACC acc = 0;
-        for (int s = 0; s < g.nshot; ++s) acc += (ACC)terms[(long long)s * b.cells + h];
-        geps[out] += acc;
-    }
-    if (want_sig) {
-        ACC acc = 0;
-        for (int s = 0; s < g.nshot; ++s) acc += (ACC)terms[sstride + (long long)s * b.cells + h];
-        gsig[out] += acc;
-    }
+    reduce_terms<ACC>(g, b, terms, h, want_eps, want_sig, geps, gsig);
 }
 
 // grad_source[k, it] += per-shot values, shots in order
@@ -677,14 +702,24 @@ void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const
     else hipLaunchKernelGGL(k_tm_step_fwd<2>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it);
 }
 
-void launch_tm_step_adj(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
-                        const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it,
-                        hipStream_t st)
+template <typename ACC>
+static void launch_adj_acc(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                           const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it,
+                           hipStream_t st)
 {
     const dim3 grid = step_grid(g);
-    if (order == 8) hipLaunchKernelGGL(k_tm_step_adj<8>, grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
-    else if (order == 4) hipLaunchKernelGGL(k_tm_step_adj<4>, grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
-    else hipLaunchKernelGGL(k_tm_step_adj<2>, grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+    if (order == 8) hipLaunchKernelGGL((k_tm_step_adj<8, ACC>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+    else if (order == 4)
+        hipLaunchKernelGGL((k_tm_step_adj<4, ACC>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+    else hipLaunchKernelGGL((k_tm_step_adj<2, ACC>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+}
+
+void launch_tm_step_adj(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                        const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it, bool fp64,
+                        hipStream_t st)
+{
+    if (fp64) launch_adj_acc<double>(order, g, f, pp, src, rx, ag, it, st);
+    else launch_adj_acc<float>(order, g, f, pp, src, rx, ag, it, st);
 }
 
 void launch_tm_grad_reduce(const Tm2dGrid& g, const Box& b, const float* terms, int want_eps, int want_sig,

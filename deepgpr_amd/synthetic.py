@@ -137,3 +137,68 @@ def c3_layered_3d(nshot: int = 4, nt: int = 2000, shape=(256, 256, 192), S: int 
 
 
 WORKLOADS = {"c1": c1_small_2d, "c2": c2_fwi_2d, "c3": c3_layered_3d}
+
+
+def _layered_model(shape, gen: torch.Generator, eps_hi: float = 25.0, sig_hi: float = 0.02):
+    """Layers along x plus 1-4 spherical (circular) inclusions, eps_r in
+    [1, eps_hi], sigma in [0, sig_hi]."""
+    nx = shape[0]
+    nlay = int(torch.randint(2, 7, (1,), generator=gen))
+    bounds = sorted(torch.randint(1, nx, (nlay - 1,), generator=gen).tolist())
+    eps = torch.empty(shape, dtype=torch.float64)
+    sig = torch.empty(shape, dtype=torch.float64)
+    edges = [0] + bounds + [nx]
+    for a, b in zip(edges[:-1], edges[1:]):
+        eps[a:b] = 1.0 + (eps_hi - 1.0) * float(torch.rand(1, generator=gen, dtype=torch.float64))
+        sig[a:b] = sig_hi * float(torch.rand(1, generator=gen, dtype=torch.float64))
+    axes = [torch.arange(n, dtype=torch.float64).reshape([-1 if k == d else 1 for k in range(len(shape))])
+            for d, n in enumerate(shape)]
+    for _ in range(int(torch.randint(1, 5, (1,), generator=gen))):
+        c = [float(torch.rand(1, generator=gen, dtype=torch.float64)) * n for n in shape]
+        r = 3.0 + 0.1 * min(shape) * float(torch.rand(1, generator=gen, dtype=torch.float64))
+        inside = sum((ax - cc) ** 2 for ax, cc in zip(axes, c)) <= r * r
+        eps = torch.where(inside, 1.0 + (eps_hi - 1.0) * float(torch.rand(1, generator=gen, dtype=torch.float64)), eps)
+        sig = torch.where(inside, sig_hi * float(torch.rand(1, generator=gen, dtype=torch.float64)), sig)
+    return eps.float(), sig.float()
+
+
+def serving_stream(n: int, rate: float = 50.0, seed: int = 5051, nt_scale: float = 1.0):
+    """Mixed Poisson request stream: 80 % 2D forward-only (grids 128^2-512^2,
+    nt 1000-4000, order 2/4/8), 15 % 2D forward+gradient, 5 % 3D forward
+    (96^3-192^3, nt 1000-2000). Layered-plus-inclusion models, Ricker sources
+    at 100-500 MHz, CPML 10, dx 0.02 m; dt 3e-11 s in 2D and 2.5e-11 s in 3D.
+    ``nt_scale`` shortens every run (for quick checks)."""
+    from .serving import SolveRequest
+
+    gen = torch.Generator().manual_seed(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += float(-math.log(1.0 - float(torch.rand(1, generator=gen, dtype=torch.float64)))) / rate
+        u = float(torch.rand(1, generator=gen, dtype=torch.float64))
+        three_d = u >= 0.95
+        gradient = 0.80 <= u < 0.95
+        if three_d:
+            side = lambda: int(torch.randint(96, 193, (1,), generator=gen)) // 8 * 8  # noqa: E731
+            shape = (side(), side(), side())
+            nt = int(torch.randint(1000, 2001, (1,), generator=gen))
+            order = int([2, 4][int(torch.randint(0, 2, (1,), generator=gen))])
+            dt = 2.5e-11
+        else:
+            side = lambda: int(torch.randint(128, 513, (1,), generator=gen)) // 16 * 16  # noqa: E731
+            shape = (side(), side())
+            nt = int(torch.randint(1000, 4001, (1,), generator=gen))
+            order = int([2, 4, 8][int(torch.randint(0, 3, (1,), generator=gen))])
+            dt = 3e-11
+        nt = max(16, int(nt * nt_scale))
+        eps, sig = _layered_model(shape, gen)
+        freq = 1e8 + 4e8 * float(torch.rand(1, generator=gen, dtype=torch.float64))
+        wave = ricker(freq, nt, dt, 1.0 / freq, dtype=torch.float64).float().reshape(1, nt, 1)
+        nz = shape[2] // 2 if three_d else 0
+        src = torch.tensor([[[5, shape[1] // 2, nz]]])
+        nrx = 32
+        ry = torch.round(torch.linspace(0, shape[1] - 1, nrx)).long()
+        rx = torch.stack([torch.full_like(ry, 5), ry, torch.full_like(ry, nz)], dim=1).unsqueeze(0)
+        out.append(SolveRequest(eps, sig, wave, src, rx, 0.02, dt, fdtd_order=order, pmlthick=10,
+                                gradient=gradient, options={"mode": 2}, rid=rid, arrival=t))
+    return out

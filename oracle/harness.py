@@ -220,6 +220,39 @@ def make_problem(NX: int, NY: int, NZ: int, *, nshot: int = 1, nt: int = 12, ord
     return prob
 
 
+def workload_problem(w, shots: int, nt: int, storage_type: int = 0) -> Problem:
+    """Native-level inputs of a synthetic workload (deepgpr_amd.synthetic) for
+    its first `shots` shots and `nt` time steps: the extended model, the CPML
+    coefficients of build_pml_coeffs, the PML-shifted locations and the
+    waveform, as compute() hands them to the library."""
+    import torch
+
+    from deepgpr_amd.preprocessing.model_setup import initialization, shift_locations
+    from deepgpr_amd.solver.pml import build_pml_coeffs
+
+    src_loc, rx_loc = w.source_location[:shots], w.receiver_location[:shots]
+    prep = initialization("cpu", w.eps_r, w.sigma, None, w.source_amplitudes, src_loc, rx_loc, w.dx, w.dt, w.pml,
+                          w.fdtd_order)
+    coeffs = build_pml_coeffs(prep.er, prep.mr, w.dt, w.dx, prep.nx, prep.ny, prep.nz, prep.pmlthick, "cpu",
+                              torch.float32)[6:]
+    src = shift_locations(src_loc, prep.pmlthick, "cpu").numpy().astype(np.int32)
+    rx = shift_locations(rx_loc, prep.pmlthick, "cpu").numpy().astype(np.int32)
+    p = make_problem(prep.nx + 1, prep.ny + 1, prep.nz + 1, order=w.fdtd_order, nt=nt, nshot=shots,
+                     nsrc=src.shape[1], nrx=rx.shape[1], seed=1, random_state=False,
+                     pml=[int(v) for v in prep.pmlthick], d=(w.dx, w.dx, w.dx))
+    p.dt = w.dt
+    p.eps, p.sig, p.mu = (t.detach().numpy().astype(np.float32) for t in (prep.ere, prep.see, prep.mr))
+    p.coef_e = [coeffs[2 * f].numpy() for f in range(6)]
+    p.coef_h = [coeffs[2 * f + 1].numpy() for f in range(6)]
+    p.src_loc, p.rx_loc = src, rx
+    wave = w.source_amplitudes[:, :nt, 0].numpy().astype(np.float32)
+    p.wave = np.ascontiguousarray(np.broadcast_to(wave, (src.shape[1], nt)))
+    p.mode, p.S = w.mode, w.sampling_interval
+    p.storage_type = storage_type
+    p.data_grad = np.random.default_rng(0).standard_normal((shots, nt, rx.shape[1])).astype(np.float32)
+    return p
+
+
 # --------------------------------------------------------------------- CPU runs
 
 
