@@ -97,6 +97,11 @@ __device__ __forceinline__ Cpml4 cpml_at(const float* R, int p, int q)
 
 // ---------------------------------------------------------------------------
 // Forward: H half step (tile + halo), E half step, sources, history, receivers
+//
+// Wavefront w handles tile rows w, w + NW, ... of every phase; lane = column.
+// Row indices, row pointers and x-face checks are therefore wave-uniform
+// (scalar); per-lane column data (the halo column a lane also covers, y-face
+// indices) is computed once.
 // ---------------------------------------------------------------------------
 template <int ORDER>
 __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
@@ -104,15 +109,17 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
 {
     using T = Tile<ORDER>;
     constexpr int R = T::R, HE = T::HE, LR = T::LR, LC = T::LC, HR = T::HR, HC = T::HC;
-    constexpr int KS = cdiv(LR * LC, NTF), KY = cdiv(HR * BJ, NTF), KX = cdiv(BI * HC, NTF),
-                  KE = cdiv(BI * BJ, NTF);
+    constexpr int NW = NTF / 64;
+    constexpr int KS = cdiv(LR, NW), KY = cdiv(HR, NW), KT = cdiv(BI, NW);
     __shared__ float sE[LR * LC];
     __shared__ float sHy[HR * BJ];
     __shared__ float sHx[BI * HC];
     __shared__ float sEn[BI * BJ];
     const int tile = step_tile(g.ftiles);
     if (tile < 0) return;
-    const int s = blockIdx.y, tid = threadIdx.x;
+    const int s = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int ti = tile / g.ntj, tj = tile - ti * g.ntj;
     const int i0 = ti * BI, j0 = tj * BJ;
     const long long sb0 = (long long)s * g.plane;
@@ -120,62 +127,83 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
     const float* __restrict__ Hx = pp.Hx[0] + sb0;
     const float* __restrict__ Hy = pp.Hy[0] + sb0;
 
-    // Every load that does not depend on LDS is issued before the first
-    // barrier, so a workgroup waits for memory about once per phase set.
-    float st[KS];
+    // this lane's columns: the tile column gj, one staged halo column hc
+    // (left: 0..HE-1, right: HE+BJ..LC-1) and one Hx-strip halo column xc
+    const int gj = j0 + lane;
+    const bool col_ok = gj < g.NY;
+    const int hc = lane < HE ? lane : (lane < 2 * HE ? BJ + lane : -1);
+    const int hgj = j0 - HE + hc;
+    const bool h_ok = hc >= 0 && hgj >= 0 && hgj < g.NY;
+    const int xc = lane < R ? lane : (lane < 2 * R - 1 ? BJ + lane : -1);
+    const int xgj = j0 - R + xc;
+    const bool x_ok = xc >= 0 && xgj >= 0 && xgj < g.NY;
+    const bool col_in = gj > 0 && gj < g.NY - 1;
+    const bool xcol_in = xgj > 0 && xgj < g.NY - 1;
+    int qe[2], qh[2], qx[2];  // y-face depth indices of gj (E, H) and xgj (H)
+#pragma unroll
+    for (int f2 = 0; f2 < 2; ++f2) {
+        qe[f2] = face_q2(g, true, 2 + f2, gj);
+        qh[f2] = face_q2(g, false, 2 + f2, gj);
+        qx[f2] = face_q2(g, false, 2 + f2, xgj);
+    }
+
+    // Every load that does not depend on LDS is issued before the first barrier.
+    float st_m[KS], st_h[KS];
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
-        const int e = tid + k * NTF;
-        const int r = e / LC, c = e - (e / LC) * LC;
-        const int gi = i0 - HE + r, gj = j0 - HE + c;
-        st[k] = (e < LR * LC && gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY) ? Ez[gi * g.NY + gj] : 0.0f;
+        const int r = w + NW * k;
+        const int gi = i0 - HE + r;
+        const bool row = r < LR && gi >= 0 && gi < g.NX;
+        st_m[k] = (row && col_ok) ? Ez[gi * g.NY + gj] : 0.0f;
+        st_h[k] = (row && h_ok) ? Ez[gi * g.NY + hgj] : 0.0f;
     }
     float y_h[KY], y_u[KY];  // Hy and hcurl of the Hy strip
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
-        const int e = tid + k * NTF;
-        const int gi = i0 - R + e / BJ, gj = j0 + e % BJ;
-        const bool ok = e < HR * BJ && gi >= 0 && gi < g.NX && gj < g.NY;
+        const int r = w + NW * k;
+        const int gi = i0 - R + r;
+        const bool ok = r < HR && gi >= 0 && gi < g.NX && col_ok;
         y_h[k] = ok ? Hy[gi * g.NY + gj] : 0.0f;
         y_u[k] = ok ? f.hcurl[gi * g.NY + gj] : 0.0f;
     }
-    float x_h[KX], x_u[KX];  // Hx and hcurl of the Hx strip
+    float x_h[KT], x_u[KT], xe_h[KT], xe_u[KT], t_ca[KT], t_cc[KT], t_cb[KT];
 #pragma unroll
-    for (int k = 0; k < KX; ++k) {
-        const int e = tid + k * NTF;
-        const int gi = i0 + e / HC, gj = j0 - R + e % HC;
-        const bool ok = e < BI * HC && gi < g.NX && gj >= 0 && gj < g.NY;
-        x_h[k] = ok ? Hx[gi * g.NY + gj] : 0.0f;
-        x_u[k] = ok ? f.hcurl[gi * g.NY + gj] : 0.0f;
-    }
-    float t_ca[KE], t_cc[KE], t_cb[KE];  // coefficients of the tile cells
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int e = tid + k * NTF;
-        const int gi = i0 + e / BJ, gj = j0 + e % BJ;
-        const bool ok = e < BI * BJ && gi < g.NX && gj < g.NY;
-        t_ca[k] = ok ? f.ca[gi * g.NY + gj] : 0.0f;
-        t_cc[k] = ok ? f.ccurl[gi * g.NY + gj] : 0.0f;
-        t_cb[k] = ok ? f.cb[gi * g.NY + gj] : 0.0f;
+    for (int k = 0; k < KT; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 + r;
+        const bool row = r < BI && gi < g.NX;
+        const bool ok = row && col_ok, okx = row && x_ok;
+        const int p = gi * g.NY + gj, px = gi * g.NY + xgj;
+        x_h[k] = ok ? Hx[p] : 0.0f;
+        x_u[k] = ok ? f.hcurl[p] : 0.0f;
+        t_ca[k] = ok ? f.ca[p] : 0.0f;
+        t_cc[k] = ok ? f.ccurl[p] : 0.0f;
+        t_cb[k] = ok ? f.cb[p] : 0.0f;
+        xe_h[k] = okx ? Hx[px] : 0.0f;
+        xe_u[k] = okx ? f.hcurl[px] : 0.0f;
     }
 #pragma unroll
-    for (int k = 0; k < KS; ++k)
-        if (tid + k * NTF < LR * LC) sE[tid + k * NTF] = st[k];
+    for (int k = 0; k < KS; ++k) {
+        const int r = w + NW * k;
+        if (r < LR) {
+            sE[r * LC + HE + lane] = st_m[k];
+            if (hc >= 0) sE[r * LC + hc] = st_h[k];
+        }
+    }
     __syncthreads();
 
     // Hy on rows [i0-R, i0+BI+R-1) of the tile's columns (x faces: H phi1)
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
-        const int e = tid + k * NTF;
-        const int r = e / BJ, c = e % BJ;
-        const int gi = i0 - R + r, gj = j0 + c;
-        if (e >= HR * BJ || gi < 0 || gi >= g.NX || gj >= g.NY) continue;
+        const int r = w + NW * k;
+        const int gi = i0 - R + r;
+        if (r >= HR || gi < 0 || gi >= g.NX || !col_ok) continue;
         const int p = gi * g.NY + gj;
         const bool own = gi >= i0 && gi < i0 + BI;
-        const int le = (r + R - 1) * LC + (c + HE);  // staged index of (gi, gj)
+        const int le = (r + R - 1) * LC + HE + lane;  // staged index of (gi, gj)
         float hy = y_h[k];
         const float u1 = y_u[k];
-        if (hy_up(g, gi, gj)) {
+        if ((g.NY - 1) != 1 && gi < g.NX - 1 && col_in) {
             const float uz = axis_k(g, u1, 2);
             const float dEz = dfwd<ORDER, int>(sE, le, LC, gi, g.NX);
             hy = hy - uz * 0.0f + u1 * dEz;
@@ -191,39 +219,63 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
             hy += f.hrhs[p] * (kc.ra1 * der + kc.rb * phi);
             if (own) pp.psi_h[1][face][pi] = kc.re * phi - kc.rf * der;
         }
-        sHy[e] = hy;
+        sHy[r * BJ + lane] = hy;
         if (own) pp.Hy[1][sb0 + p] = hy;
     }
-    // Hx on columns [j0-R, j0+BJ+R-1) of the tile's rows (y faces: H phi1)
+    // Hx on the tile's rows: the tile columns, then the strip's halo columns (y faces: H phi1)
 #pragma unroll
-    for (int k = 0; k < KX; ++k) {
-        const int e = tid + k * NTF;
-        const int r = e / HC, c = e - (e / HC) * HC;
-        const int gi = i0 + r, gj = j0 - R + c;
-        if (e >= BI * HC || gi >= g.NX || gj < 0 || gj >= g.NY) continue;
-        const int p = gi * g.NY + gj;
-        const bool own = gj >= j0 && gj < j0 + BJ;
-        const int le = (r + HE) * LC + (c + R - 1);
-        float hx = x_h[k];
-        const float u1 = x_u[k];
-        if (hx_up(g, gi, gj)) {
-            const float uy = axis_k(g, u1, 1), uz = axis_k(g, u1, 2);
-            const float dEz = dfwd<ORDER, int>(sE, le, 1, gj, g.NY);
-            hx = hx - uy * dEz + uz * 0.0f;
-        }
+    for (int k = 0; k < KT; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 + r;
+        if (r >= BI || gi >= g.NX) continue;
+        const bool row_up = (g.NX - 1) != 1 && gi > 0 && gi < g.NX - 1;
+        if (col_ok) {
+            const int p = gi * g.NY + gj;
+            const int le = (r + HE) * LC + HE + lane;
+            float hx = x_h[k];
+            const float u1 = x_u[k];
+            if (row_up && gj < g.NY - 1) {
+                const float uy = axis_k(g, u1, 1), uz = axis_k(g, u1, 2);
+                const float dEz = dfwd<ORDER, int>(sE, le, 1, gj, g.NY);
+                hx = hx - uy * dEz + uz * 0.0f;
+            }
 #pragma unroll
-        for (int face = 2; face < 4; ++face) {
-            const int q = face_q2(g, false, face, gj);
-            if (q < 0) continue;
-            const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
-            const float der = dfwd<ORDER, int>(sE, le, 1, gj, g.NY) / g.d[1];
-            const long long pi = psi_at(g, false, face, s, gi, gj, q);
-            const float phi = pp.psi_h[0][face][pi];
-            hx -= f.hrhs[p] * (kc.ra1 * der + kc.rb * phi);
-            if (own) pp.psi_h[1][face][pi] = kc.re * phi - kc.rf * der;
+            for (int f2 = 0; f2 < 2; ++f2) {
+                const int q = qh[f2];
+                if (q < 0) continue;
+                const int face = 2 + f2;
+                const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
+                const float der = dfwd<ORDER, int>(sE, le, 1, gj, g.NY) / g.d[1];
+                const long long pi = psi_at(g, false, face, s, gi, gj, q);
+                const float phi = pp.psi_h[0][face][pi];
+                hx -= f.hrhs[p] * (kc.ra1 * der + kc.rb * phi);
+                pp.psi_h[1][face][pi] = kc.re * phi - kc.rf * der;
+            }
+            sHx[r * HC + R + lane] = hx;
+            pp.Hx[1][sb0 + p] = hx;
         }
-        sHx[e] = hx;
-        if (own) pp.Hx[1][sb0 + p] = hx;
+        if (x_ok) {
+            const int p = gi * g.NY + xgj;
+            const int le = (r + HE) * LC + (xc - R + HE);
+            float hx = xe_h[k];
+            const float u1 = xe_u[k];
+            if (row_up && xgj < g.NY - 1) {
+                const float uy = axis_k(g, u1, 1), uz = axis_k(g, u1, 2);
+                const float dEz = dfwd<ORDER, int>(sE, le, 1, xgj, g.NY);
+                hx = hx - uy * dEz + uz * 0.0f;
+            }
+#pragma unroll
+            for (int f2 = 0; f2 < 2; ++f2) {
+                const int q = qx[f2];
+                if (q < 0) continue;
+                const int face = 2 + f2;
+                const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
+                const float der = dfwd<ORDER, int>(sE, le, 1, xgj, g.NY) / g.d[1];
+                const float phi = pp.psi_h[0][face][psi_at(g, false, face, s, gi, xgj, q)];
+                hx -= f.hrhs[p] * (kc.ra1 * der + kc.rb * phi);
+            }
+            sHx[r * HC + xc] = hx;
+        }
     }
     __syncthreads();
 
@@ -233,17 +285,18 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
         s_beg = src.block_off[tile + g.ftiles * s];
         s_end = src.block_off[tile + g.ftiles * s + 1];
     }
+    const bool ez_deg = (g.NX - 1) != 1 || (g.NY - 1) != 1;
 #pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int e = tid + k * NTF;
-        const int r = e / BJ, c = e % BJ;
-        const int gi = i0 + r, gj = j0 + c;
-        if (e >= BI * BJ || gi >= g.NX || gj >= g.NY) continue;
+    for (int k = 0; k < KT; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 + r;
+        if (r >= BI || gi >= g.NX || !col_ok) continue;
         const int p = gi * g.NY + gj;
-        const float eold = sE[(r + HE) * LC + (c + HE)];
-        const int ly = (r + R) * BJ + c;  // sHy index of (gi, gj)
-        const int lx = r * HC + (c + R);  // sHx index of (gi, gj)
-        const bool up = ez_up(g, gi, gj);
+        const int e = r * BJ + lane;
+        const float eold = sE[(r + HE) * LC + HE + lane];
+        const int ly = (r + R) * BJ + lane;  // sHy index of (gi, gj)
+        const int lx = r * HC + R + lane;    // sHx index of (gi, gj)
+        const bool up = ez_deg && gi > 0 && gi < g.NX - 1 && col_in;
         const float u0 = t_ca[k];
         float ez = eold;
         if (up) {
@@ -256,7 +309,7 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
         const float cbv = t_cb[k];
 #pragma unroll
         for (int face = 0; face < 4; ++face) {
-            const int q = face_q2(g, true, face, face < 2 ? gi : gj);
+            const int q = face < 2 ? face_q2(g, true, face, gi) : qe[face - 2];
             if (q < 0) continue;
             const Cpml4 kc = cpml_at(f.coef_e[face], g.pml[face], q);
             const float der = face < 2 ? dbwd<ORDER, int>(sHy, ly, BJ, gi, g.NX) / g.d[0]
@@ -299,7 +352,7 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
     if (rx.count > 0) {
         __syncthreads();
         const int b0 = rx.off[tile + g.ftiles * s], b1 = rx.off[tile + g.ftiles * s + 1];
-        for (int q = b0 + tid; q < b1; q += NTF) {
+        for (int q = b0 + (int)threadIdx.x; q < b1; q += NTF) {
             const int2 ent = rx.ent[q];
             rx.out[((long long)s * rx.nt + it) * rx.nrx + ent.y] = sEn[ent.x];
         }
@@ -309,6 +362,24 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
 // ---------------------------------------------------------------------------
 // Adjoint: receivers^T, source and material terms, (E step)^T, (H step)^T
 // ---------------------------------------------------------------------------
+// sum over s of v[s * stride], in shot order; the loads of 8 shots are issued
+// before their adds so a cell waits for memory once per 8 shots.
+template <typename ACC>
+__device__ __forceinline__ ACC sum_shots(const float* __restrict__ v, long long stride, int nshot)
+{
+    ACC acc = 0;
+    int s = 0;
+    for (; s + 8 <= nshot; s += 8) {
+        float x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = v[(long long)(s + q) * stride];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += (ACC)x[q];
+    }
+    for (; s < nshot; ++s) acc += (ACC)v[(long long)s * stride];
+    return acc;
+}
+
 // Sum of the per-shot material terms of cell h of the history box, in shot
 // order, added to the accumulators (one saved step).
 template <typename ACC>
@@ -319,16 +390,8 @@ __device__ __forceinline__ void reduce_terms(const Tm2dGrid& g, const Box& b, co
     const int i = (int)(h / b.ny) + b.x0, j = (int)(h % b.ny) + b.y0;
     const long long out = (long long)i * (g.NY - 1) + j;
     const long long sstride = (long long)g.nshot * b.cells;
-    if (want_eps) {
-        ACC acc = 0;
-        for (int s = 0; s < g.nshot; ++s) acc += (ACC)terms[(long long)s * b.cells + h];
-        geps[out] += acc;
-    }
-    if (want_sig) {
-        ACC acc = 0;
-        for (int s = 0; s < g.nshot; ++s) acc += (ACC)terms[sstride + (long long)s * b.cells + h];
-        gsig[out] += acc;
-    }
+    if (want_eps) geps[out] += sum_shots<ACC>(terms + h, b.cells, g.nshot);
+    if (want_sig) gsig[out] += sum_shots<ACC>(terms + sstride + h, b.cells, g.nshot);
 }
 
 template <int ORDER, typename ACC>
@@ -337,16 +400,18 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
 {
     using T = Tile<ORDER>;
     constexpr int R = T::R, HE = T::HE, LR = T::LR, LC = T::LC, HR = T::HR, HC = T::HC;
-    constexpr int KS = cdiv(LR * LC, NTA), KA = cdiv(LR * BJ, NTA), KB = cdiv(BI * LC, NTA),
-                  KY = cdiv(HR * BJ, NTA), KX = cdiv(BI * HC, NTA), KE = cdiv(BI * BJ, NTA);
+    constexpr int NW = NTA / 64;
+    constexpr int KS = cdiv(LR, NW), KY = cdiv(HR, NW), KT = cdiv(BI, NW);
     __shared__ float sL[LR * LC];   // lambda_Ez after receivers^T
     __shared__ float sA[LR * BJ];   // (Hy, x) weights of (E step)^T, then (Ez, x) weights of (H step)^T
-    __shared__ float sB[BI * LC];   // (Hx, y) weights, then (Ez, y) weights
+    __shared__ float sB[BI * LC];   // (Hx, y) weights (row stride LC), then (Ez, y) weights (row stride HC)
     __shared__ float sHy[HR * BJ];  // new lambda_Hy on rows [i0-R, i0+BI+R-1)
     __shared__ float sHx[BI * HC];  // new lambda_Hx on columns [j0-R, j0+BJ+R-1)
     const int tile = step_tile(g.ftiles);
     if (tile < 0) return;
-    const int s = blockIdx.y, tid = threadIdx.x;
+    const int s = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int ti = tile / g.ntj, tj = tile - ti * g.ntj;
     const int i0 = ti * BI, j0 = tj * BJ;
     const int lt = tile + g.ftiles * s;
@@ -355,68 +420,66 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
     const float* __restrict__ Lx = pp.Hx[0] + sb0;
     const float* __restrict__ Ly = pp.Hy[0] + sb0;
 
+    // this lane's columns: tile column gj, staged halo column hc, Hx-strip halo column xc
+    const int gj = j0 + lane;
+    const bool col_ok = gj < g.NY;
+    const int hc = lane < HE ? lane : (lane < 2 * HE ? BJ + lane : -1);
+    const int hgj = j0 - HE + hc;
+    const bool h_ok = hc >= 0 && hgj >= 0 && hgj < g.NY;
+    const int xc = lane < R ? lane : (lane < 2 * R - 1 ? BJ + lane : -1);
+    const int xgj = j0 - R + xc;
+    const bool x_ok = xc >= 0 && xgj >= 0 && xgj < g.NY;
+    const bool ez_deg = (g.NX - 1) != 1 || (g.NY - 1) != 1;
+    int qe[2], qeh[2], qh[2], qx[2];  // y-face depth indices: E at gj / hgj, H at gj / xgj
+#pragma unroll
+    for (int f2 = 0; f2 < 2; ++f2) {
+        qe[f2] = face_q2(g, true, 2 + f2, gj);
+        qeh[f2] = face_q2(g, true, 2 + f2, hgj);
+        qh[f2] = face_q2(g, false, 2 + f2, gj);
+        qx[f2] = face_q2(g, false, 2 + f2, xgj);
+    }
+
     // Every load that does not depend on LDS is issued before the first barrier.
-    float st[KS];
+    float st_m[KS], st_h[KS], a_cc[KS];
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
-        const int e = tid + k * NTA;
-        const int r = e / LC, c = e - (e / LC) * LC;
-        const int gi = i0 - HE + r, gj = j0 - HE + c;
-        st[k] = (e < LR * LC && gi >= 0 && gi < g.NX && gj >= 0 && gj < g.NY) ? Lz[gi * g.NY + gj] : 0.0f;
-    }
-    float a_cc[KA];  // ccurl of the x-weight rows
-#pragma unroll
-    for (int k = 0; k < KA; ++k) {
-        const int e = tid + k * NTA;
-        const int gi = i0 - HE + e / BJ, gj = j0 + e % BJ;
-        a_cc[k] = (e < LR * BJ && gi >= 0 && gi < g.NX && gj < g.NY) ? f.ccurl[gi * g.NY + gj] : 0.0f;
-    }
-    float b_cc[KB];  // ccurl of the y-weight columns
-#pragma unroll
-    for (int k = 0; k < KB; ++k) {
-        const int e = tid + k * NTA;
-        const int gi = i0 + e / LC, gj = j0 - HE + e % LC;
-        b_cc[k] = (e < BI * LC && gi < g.NX && gj >= 0 && gj < g.NY) ? f.ccurl[gi * g.NY + gj] : 0.0f;
+        const int r = w + NW * k;
+        const int gi = i0 - HE + r;
+        const bool row = r < LR && gi >= 0 && gi < g.NX;
+        st_m[k] = (row && col_ok) ? Lz[gi * g.NY + gj] : 0.0f;
+        st_h[k] = (row && h_ok) ? Lz[gi * g.NY + hgj] : 0.0f;
+        a_cc[k] = (row && col_ok) ? f.ccurl[gi * g.NY + gj] : 0.0f;
     }
     float y_l[KY], y_u[KY];  // lambda_Hy and hcurl of the Hy strip
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
-        const int e = tid + k * NTA;
-        const int gi = i0 - R + e / BJ, gj = j0 + e % BJ;
-        const bool ok = e < HR * BJ && gi >= 0 && gi < g.NX && gj < g.NY;
+        const int r = w + NW * k;
+        const int gi = i0 - R + r;
+        const bool ok = r < HR && gi >= 0 && gi < g.NX && col_ok;
         y_l[k] = ok ? Ly[gi * g.NY + gj] : 0.0f;
         y_u[k] = ok ? f.hcurl[gi * g.NY + gj] : 0.0f;
     }
-    float x_l[KX], x_u[KX];  // lambda_Hx and hcurl of the Hx strip
-#pragma unroll
-    for (int k = 0; k < KX; ++k) {
-        const int e = tid + k * NTA;
-        const int gi = i0 + e / HC, gj = j0 - R + e % HC;
-        const bool ok = e < BI * HC && gi < g.NX && gj >= 0 && gj < g.NY;
-        x_l[k] = ok ? Lx[gi * g.NY + gj] : 0.0f;
-        x_u[k] = ok ? f.hcurl[gi * g.NY + gj] : 0.0f;
-    }
-    float t_ca[KE], t_cb[KE];  // coefficients of the tile cells
-#pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int e = tid + k * NTA;
-        const int gi = i0 + e / BJ, gj = j0 + e % BJ;
-        const bool ok = e < BI * BJ && gi < g.NX && gj < g.NY;
-        t_ca[k] = ok ? f.ca[gi * g.NY + gj] : 0.0f;
-        t_cb[k] = ok ? f.cb[gi * g.NY + gj] : 0.0f;
-    }
-    // history of a saved step (tile cells inside the history box)
     const MatGrad& mg = ag.mg;
     const Box& hb = mg.box;
-    float m_e[KE], m_s[KE], m_sig[KE];
+    float x_l[KT], x_u[KT], xe_l[KT], xe_u[KT], b_cm[KT], b_ch[KT], t_ca[KT], t_cb[KT];
+    float m_e[KT], m_s[KT], m_sig[KT];  // history of a saved step and sigma (tile cells in the box)
 #pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int e = tid + k * NTA;
-        const int gi = i0 + e / BJ, gj = j0 + e % BJ;
-        const bool ok = ag.terms && e < BI * BJ && gi < g.NX && gj < g.NY && gi >= hb.x0 && gi < hb.x0 + hb.nx &&
-                        gj >= hb.y0 && gj < hb.y0 + hb.ny;
+    for (int k = 0; k < KT; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 + r;
+        const bool row = r < BI && gi < g.NX;
+        const bool ok = row && col_ok, okx = row && x_ok, okh = row && h_ok;
+        const int p = gi * g.NY + gj, px = gi * g.NY + xgj;
+        x_l[k] = ok ? Lx[p] : 0.0f;
+        x_u[k] = ok ? f.hcurl[p] : 0.0f;
+        b_cm[k] = ok ? f.ccurl[p] : 0.0f;
+        t_ca[k] = ok ? f.ca[p] : 0.0f;
+        t_cb[k] = ok ? f.cb[p] : 0.0f;
+        xe_l[k] = okx ? Lx[px] : 0.0f;
+        xe_u[k] = okx ? f.hcurl[px] : 0.0f;
+        b_ch[k] = okh ? f.ccurl[gi * g.NY + hgj] : 0.0f;
         m_e[k] = m_s[k] = m_sig[k] = 0.0f;
-        if (ok) {
+        if (ag.terms && ok && gi >= hb.x0 && gi < hb.x0 + hb.nx && gj >= hb.y0 && gj < hb.y0 + hb.ny) {
             const long long base = (long long)s * hb.cells + (long long)(gi - hb.x0) * hb.ny + (gj - hb.y0);
             m_e[k] = load_hist(mg.eframe, ag.ez_comp * mg.e_cstride + base, mg.kind);
             m_s[k] = load_hist(mg.sframe, ag.ez_comp * mg.s_cstride + base, mg.kind);
@@ -424,12 +487,17 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
         }
     }
 #pragma unroll
-    for (int k = 0; k < KS; ++k)
-        if (tid + k * NTA < LR * LC) sL[tid + k * NTA] = st[k];
+    for (int k = 0; k < KS; ++k) {
+        const int r = w + NW * k;
+        if (r < LR) {
+            sL[r * LC + HE + lane] = st_m[k];
+            if (hc >= 0) sL[r * LC + hc] = st_h[k];
+        }
+    }
     __syncthreads();
     if (rx.count > 0) {
         const int b0 = rx.off[lt], b1 = rx.off[lt + 1];
-        for (int q = b0 + tid; q < b1; q += NTA) {
+        for (int q = b0 + (int)threadIdx.x; q < b1; q += NTA) {
             const int4 grp = rx.grp[q];
             float acc = sL[grp.x];
             for (int m = grp.y; m < grp.y + grp.z; ++m)
@@ -441,7 +509,7 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
     // source^T: one value per (step, shot, source), summed over shots later
     if (ag.src_vals && src.count > 0) {
         const int q0 = src.block_off[lt], q1 = src.block_off[lt + 1];
-        for (int q = q0 + tid; q < q1; q += NTA) {
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += NTA) {
             const int2 ent = src.entries[q];
             const int r = ent.x / BJ, c = ent.x % BJ;
             const int p = (i0 + r) * g.NY + (j0 + c);
@@ -454,18 +522,17 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
         const long long sstride = (long long)g.nshot * hb.cells;
         const float wgt = (float)mg.weight;
 #pragma unroll
-        for (int k = 0; k < KE; ++k) {
-            const int e = tid + k * NTA;
-            const int r = e / BJ, c = e % BJ;
-            const int gi = i0 + r, gj = j0 + c;
-            if (e >= BI * BJ || gi >= g.NX || gj >= g.NY) continue;
+        for (int k = 0; k < KT; ++k) {
+            const int r = w + NW * k;
+            const int gi = i0 + r;
+            if (r >= BI || gi >= g.NX || !col_ok) continue;
             if (gi < hb.x0 || gi >= hb.x0 + hb.nx || gj < hb.y0 || gj >= hb.y0 + hb.ny) continue;
             const long long base = (long long)s * hb.cells + (long long)(gi - hb.x0) * hb.ny + (gj - hb.y0);
             const float cav = t_ca[k], cbv = t_cb[k];
-            const bool up = ez_up(g, gi, gj);
+            const bool up = ez_deg && gi > 0 && gi < g.NX - 1 && gj > 0 && gj < g.NY - 1;
             const float e_old = m_e[k];
             const float sec = m_s[k];
-            float lam = sL[(r + HE) * LC + (c + HE)];
+            float lam = sL[(r + HE) * LC + HE + lane];
             float rhs = sec;
             if (mg.e_only) {
                 const float a = up ? cav : 1.0f;
@@ -496,15 +563,14 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
         }
     }
 
-    // (E step)^T weights on rows [i0-HE, i0+BI+HE); owners advance lambda psi_E of the x faces
+    // (E step)^T weights: x channel on the staged rows (owners advance lambda psi_E of the x faces)
 #pragma unroll
-    for (int k = 0; k < KA; ++k) {
-        const int e = tid + k * NTA;
-        const int r = e / BJ, c = e % BJ;
-        const int gi = i0 - HE + r, gj = j0 + c;
-        if (e >= LR * BJ || gi < 0 || gi >= g.NX || gj >= g.NY) continue;
+    for (int k = 0; k < KS; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 - HE + r;
+        if (r >= LR || gi < 0 || gi >= g.NX || !col_ok) continue;
         const int p = gi * g.NY + gj;
-        const float lz = sL[r * LC + (c + HE)];
+        const float lz = sL[r * LC + HE + lane];
         const bool own = gi >= i0 && gi < i0 + BI;
         float acc = 0.0f;
 #pragma unroll
@@ -519,74 +585,98 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
             acc += (1.0f * u * kc.ra1 * lz - kc.rf * old) * inv;
             if (own) pp.psi_e[1][face][pi] = 1.0f * u * kc.rb * lz + kc.re * old;
         }
-        if (ez_up(g, gi, gj)) acc += axis_k(g, a_cc[k], 0) * lz;
-        sA[e] = acc;
+        if (ez_deg && gi > 0 && gi < g.NX - 1 && gj > 0 && gj < g.NY - 1) acc += axis_k(g, a_cc[k], 0) * lz;
+        sA[r * BJ + lane] = acc;
     }
-    // ... on columns [j0-HE, j0+BJ+HE); owners advance lambda psi_E of the y faces
+    // ... y channel on the tile rows: tile columns (owners advance lambda psi_E of the y faces), then halo columns
 #pragma unroll
-    for (int k = 0; k < KB; ++k) {
-        const int e = tid + k * NTA;
-        const int r = e / LC, c = e - (e / LC) * LC;
-        const int gi = i0 + r, gj = j0 - HE + c;
-        if (e >= BI * LC || gi >= g.NX || gj < 0 || gj >= g.NY) continue;
-        const int p = gi * g.NY + gj;
-        const float lz = sL[(r + HE) * LC + c];
-        const bool own = gj >= j0 && gj < j0 + BJ;
-        float acc = 0.0f;
+    for (int k = 0; k < KT; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 + r;
+        if (r >= BI || gi >= g.NX) continue;
+        const bool row_in = ez_deg && gi > 0 && gi < g.NX - 1;
+        if (col_ok) {
+            const int p = gi * g.NY + gj;
+            const float lz = sL[(r + HE) * LC + HE + lane];
+            float acc = 0.0f;
 #pragma unroll
-        for (int face = 2; face < 4; ++face) {
-            const int q = face_q2(g, true, face, gj);
-            if (q < 0) continue;
-            const Cpml4 kc = cpml_at(f.coef_e[face], g.pml[face], q);
-            const long long pi = psi_at(g, true, face, s, gi, gj, q);
-            const float old = pp.psi_e[0][face][pi];
-            const float u = f.cb[p];
-            const float inv = 1.0f / g.d[1];
-            acc += (-1.0f * u * kc.ra1 * lz - kc.rf * old) * inv;
-            if (own) pp.psi_e[1][face][pi] = -1.0f * u * kc.rb * lz + kc.re * old;
+            for (int f2 = 0; f2 < 2; ++f2) {
+                const int q = qe[f2];
+                if (q < 0) continue;
+                const int face = 2 + f2;
+                const Cpml4 kc = cpml_at(f.coef_e[face], g.pml[face], q);
+                const long long pi = psi_at(g, true, face, s, gi, gj, q);
+                const float old = pp.psi_e[0][face][pi];
+                const float u = f.cb[p];
+                const float inv = 1.0f / g.d[1];
+                acc += (-1.0f * u * kc.ra1 * lz - kc.rf * old) * inv;
+                pp.psi_e[1][face][pi] = -1.0f * u * kc.rb * lz + kc.re * old;
+            }
+            if (row_in && gj > 0 && gj < g.NY - 1) acc += -axis_k(g, b_cm[k], 1) * lz;
+            sB[r * LC + HE + lane] = acc;
         }
-        if (ez_up(g, gi, gj)) acc += -axis_k(g, b_cc[k], 1) * lz;
-        sB[e] = acc;
+        if (h_ok) {
+            const int p = gi * g.NY + hgj;
+            const float lz = sL[(r + HE) * LC + hc];
+            float acc = 0.0f;
+#pragma unroll
+            for (int f2 = 0; f2 < 2; ++f2) {
+                const int q = qeh[f2];
+                if (q < 0) continue;
+                const int face = 2 + f2;
+                const Cpml4 kc = cpml_at(f.coef_e[face], g.pml[face], q);
+                const float old = pp.psi_e[0][face][psi_at(g, true, face, s, gi, hgj, q)];
+                const float u = f.cb[p];
+                const float inv = 1.0f / g.d[1];
+                acc += (-1.0f * u * kc.ra1 * lz - kc.rf * old) * inv;
+            }
+            if (row_in && hgj > 0 && hgj < g.NY - 1) acc += -axis_k(g, b_ch[k], 1) * lz;
+            sB[r * LC + hc] = acc;
+        }
     }
     __syncthreads();
 
-    // lambda_Hy on rows [i0-R, i0+BI+R-1), lambda_Hx on columns [j0-R, j0+BJ+R-1)
+    // lambda_Hy on rows [i0-R, i0+BI+R-1): the (Hy, x) channel, then the identically zero (Hy, z) channel
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
-        const int e = tid + k * NTA;
-        const int r = e / BJ, c = e % BJ;
-        const int gi = i0 - R + r, gj = j0 + c;
-        if (e >= HR * BJ || gi < 0 || gi >= g.NX || gj >= g.NY) continue;
-        const int base = (r + R - 1) * BJ + c;  // sA row of gi is gi - (i0 - HE)
+        const int r = w + NW * k;
+        const int gi = i0 - R + r;
+        if (r >= HR || gi < 0 || gi >= g.NX || !col_ok) continue;
+        const int base = (r + R - 1) * BJ + lane;  // sA row of gi is gi - (i0 - HE)
         auto wx = [&](int off) { return sA[base + off * BJ]; };
-        // the (Hy, x) channel, then the identically zero (Hy, z) channel
         const float v = (y_l[k] + gather_bwd_T<ORDER>(wx, gi, g.NX)) + 0.0f;
-        sHy[e] = v;
+        sHy[r * BJ + lane] = v;
         if (gi >= i0 && gi < i0 + BI) pp.Hy[1][sb0 + gi * g.NY + gj] = v;
     }
+    // lambda_Hx on the tile rows: tile columns, then the strip's halo columns
 #pragma unroll
-    for (int k = 0; k < KX; ++k) {
-        const int e = tid + k * NTA;
-        const int r = e / HC, c = e - (e / HC) * HC;
-        const int gi = i0 + r, gj = j0 - R + c;
-        if (e >= BI * HC || gi >= g.NX || gj < 0 || gj >= g.NY) continue;
-        const int base = r * LC + (c + R - 1);  // sB column of gj is gj - (j0 - HE)
-        auto wy = [&](int off) { return sB[base + off]; };
-        const float v = (x_l[k] + gather_bwd_T<ORDER>(wy, gj, g.NY)) + 0.0f;
-        sHx[e] = v;
-        if (gj >= j0 && gj < j0 + BJ) pp.Hx[1][sb0 + gi * g.NY + gj] = v;
+    for (int k = 0; k < KT; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 + r;
+        if (r >= BI || gi >= g.NX) continue;
+        if (col_ok) {
+            const int base = r * LC + HE + lane;  // sB column of gj is gj - (j0 - HE)
+            auto wy = [&](int off) { return sB[base + off]; };
+            const float v = (x_l[k] + gather_bwd_T<ORDER>(wy, gj, g.NY)) + 0.0f;
+            sHx[r * HC + R + lane] = v;
+            pp.Hx[1][sb0 + gi * g.NY + gj] = v;
+        }
+        if (x_ok) {
+            const int base = r * LC + (xc - R + HE);
+            auto wy = [&](int off) { return sB[base + off]; };
+            sHx[r * HC + xc] = (xe_l[k] + gather_bwd_T<ORDER>(wy, xgj, g.NY)) + 0.0f;
+        }
     }
     __syncthreads();
 
-    // (H step)^T weights into sA (rows [i0-R, ...)) and sB (columns [j0-R, ...)); owners advance lambda psi_H
+    // (H step)^T weights: x channel into sA (rows [i0-R, ...)), y channel into sB (row stride HC)
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
-        const int e = tid + k * NTA;
-        const int r = e / BJ, c = e % BJ;
-        const int gi = i0 - R + r, gj = j0 + c;
-        if (e >= HR * BJ || gi < 0 || gi >= g.NX || gj >= g.NY) continue;
+        const int r = w + NW * k;
+        const int gi = i0 - R + r;
+        if (r >= HR || gi < 0 || gi >= g.NX || !col_ok) continue;
         const int p = gi * g.NY + gj;
-        const float ly = sHy[e];
+        const float ly = sHy[r * BJ + lane];
         const bool own = gi >= i0 && gi < i0 + BI;
         float acc = 0.0f;
 #pragma unroll
@@ -601,47 +691,66 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
             acc += (1.0f * u * kc.ra1 * ly - kc.rf * old) * inv;
             if (own) pp.psi_h[1][face][pi] = 1.0f * u * kc.rb * ly + kc.re * old;
         }
-        if (hy_up(g, gi, gj)) acc += axis_k(g, y_u[k], 0) * ly;
-        sA[e] = acc;
+        if ((g.NY - 1) != 1 && gi < g.NX - 1 && gj > 0 && gj < g.NY - 1) acc += axis_k(g, y_u[k], 0) * ly;
+        sA[r * BJ + lane] = acc;
     }
 #pragma unroll
-    for (int k = 0; k < KX; ++k) {
-        const int e = tid + k * NTA;
-        const int r = e / HC, c = e - (e / HC) * HC;
-        const int gi = i0 + r, gj = j0 - R + c;
-        if (e >= BI * HC || gi >= g.NX || gj < 0 || gj >= g.NY) continue;
-        const int p = gi * g.NY + gj;
-        const float lx = sHx[e];
-        const bool own = gj >= j0 && gj < j0 + BJ;
-        float acc = 0.0f;
+    for (int k = 0; k < KT; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 + r;
+        if (r >= BI || gi >= g.NX) continue;
+        const bool row_up = (g.NX - 1) != 1 && gi > 0 && gi < g.NX - 1;
+        if (col_ok) {
+            const int p = gi * g.NY + gj;
+            const float lx = sHx[r * HC + R + lane];
+            float acc = 0.0f;
 #pragma unroll
-        for (int face = 2; face < 4; ++face) {
-            const int q = face_q2(g, false, face, gj);
-            if (q < 0) continue;
-            const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
-            const long long pi = psi_at(g, false, face, s, gi, gj, q);
-            const float old = pp.psi_h[0][face][pi];
-            const float u = f.hrhs[p];
-            const float inv = 1.0f / g.d[1];
-            acc += (-1.0f * u * kc.ra1 * lx - kc.rf * old) * inv;
-            if (own) pp.psi_h[1][face][pi] = -1.0f * u * kc.rb * lx + kc.re * old;
+            for (int f2 = 0; f2 < 2; ++f2) {
+                const int q = qh[f2];
+                if (q < 0) continue;
+                const int face = 2 + f2;
+                const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
+                const long long pi = psi_at(g, false, face, s, gi, gj, q);
+                const float old = pp.psi_h[0][face][pi];
+                const float u = f.hrhs[p];
+                const float inv = 1.0f / g.d[1];
+                acc += (-1.0f * u * kc.ra1 * lx - kc.rf * old) * inv;
+                pp.psi_h[1][face][pi] = -1.0f * u * kc.rb * lx + kc.re * old;
+            }
+            if (row_up && gj < g.NY - 1) acc += -axis_k(g, x_u[k], 1) * lx;
+            sB[r * HC + R + lane] = acc;
         }
-        if (hx_up(g, gi, gj)) acc += -axis_k(g, x_u[k], 1) * lx;
-        sB[e] = acc;
+        if (x_ok) {
+            const int p = gi * g.NY + xgj;
+            const float lx = sHx[r * HC + xc];
+            float acc = 0.0f;
+#pragma unroll
+            for (int f2 = 0; f2 < 2; ++f2) {
+                const int q = qx[f2];
+                if (q < 0) continue;
+                const int face = 2 + f2;
+                const Cpml4 kc = cpml_at(f.coef_h[face], g.pml[face], q);
+                const float old = pp.psi_h[0][face][psi_at(g, false, face, s, gi, xgj, q)];
+                const float u = f.hrhs[p];
+                const float inv = 1.0f / g.d[1];
+                acc += (-1.0f * u * kc.ra1 * lx - kc.rf * old) * inv;
+            }
+            if (row_up && xgj < g.NY - 1) acc += -axis_k(g, xe_u[k], 1) * lx;
+            sB[r * HC + xc] = acc;
+        }
     }
     __syncthreads();
 
     // lambda_Ez = (ca) lambda_Ez + D+^T (x weights) + D+^T (y weights)
 #pragma unroll
-    for (int k = 0; k < KE; ++k) {
-        const int e = tid + k * NTA;
-        const int r = e / BJ, c = e % BJ;
-        const int gi = i0 + r, gj = j0 + c;
-        if (e >= BI * BJ || gi >= g.NX || gj >= g.NY) continue;
-        float acc = sL[(r + HE) * LC + (c + HE)];
-        if (ez_up(g, gi, gj)) acc = t_ca[k] * acc;
-        const int bx = (r + R) * BJ + c;  // sA row of gi is gi - (i0 - R)
-        const int by = r * HC + (c + R);  // sB column of gj is gj - (j0 - R)
+    for (int k = 0; k < KT; ++k) {
+        const int r = w + NW * k;
+        const int gi = i0 + r;
+        if (r >= BI || gi >= g.NX || !col_ok) continue;
+        float acc = sL[(r + HE) * LC + HE + lane];
+        if (ez_deg && gi > 0 && gi < g.NX - 1 && gj > 0 && gj < g.NY - 1) acc = t_ca[k] * acc;
+        const int bx = (r + R) * BJ + lane;  // sA row of gi is gi - (i0 - R)
+        const int by = r * HC + R + lane;    // sB column of gj is gj - (j0 - R)
         auto vx = [&](int off) { return sA[bx + off * BJ]; };
         auto vy = [&](int off) { return sB[by + off]; };
         acc += gather_fwd_T<ORDER>(vx, gi, g.NX);
@@ -655,11 +764,11 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
         const Box& b = ag.mg.box;
         const int chunk = (BI * BJ + g.nshot - 1) / g.nshot;
         const int e_end = min(BI * BJ, (s + 1) * chunk);
-        for (int e = s * chunk + tid; e < e_end; e += NTA) {
-            const int gi = i0 + e / BJ, gj = j0 + e % BJ;
-            if (gi >= g.NX || gj >= g.NY || gi < b.x0 || gi >= b.x0 + b.nx || gj < b.y0 || gj >= b.y0 + b.ny)
+        for (int e = s * chunk + (int)threadIdx.x; e < e_end; e += NTA) {
+            const int gi = i0 + e / BJ, gjj = j0 + e % BJ;
+            if (gi >= g.NX || gjj >= g.NY || gi < b.x0 || gi >= b.x0 + b.nx || gjj < b.y0 || gjj >= b.y0 + b.ny)
                 continue;
-            reduce_terms<ACC>(g, b, ag.pending, (long long)(gi - b.x0) * b.ny + (gj - b.y0), ag.mg.want_eps,
+            reduce_terms<ACC>(g, b, ag.pending, (long long)(gi - b.x0) * b.ny + (gjj - b.y0), ag.mg.want_eps,
                               ag.mg.want_sig, (ACC*)ag.geps, (ACC*)ag.gsig);
         }
     }
