@@ -146,12 +146,13 @@ def kernel_bytes(kernel: str, w, args) -> float:
     if kernel == "fwd_e":
         core = 24 if two_d else 48      # H read, E read+write, ca, ce_curl (+ce_rhs in 3D)
         return cells * (core + comps * hist_b * frames / w.sampling_interval) + 8 * pe
+    if kernel in ("adj_weights_e", "adj_weights_h"):
+        # lambda (3) + curl coefficient read, six channel weights written, lambda psi read+write
+        return cells * 40 + 8 * (pe if kernel == "adj_weights_e" else ph)
     if kernel == "adj_e":
-        core = 24 if two_d else 40      # lambda_E read, ce_curl, lambda_H read+write
-        return cells * core + 4 * pe
+        return cells * 48               # six channel weights read, lambda_H read+write
     if kernel == "adj_h":
-        core = 28 if two_d else 44      # lambda_H read, ch_curl, lambda_E read+write, ca
-        return cells * core + 4 * ph
+        return cells * 52               # six channel weights read, lambda_E read+write, ca
     if kernel == "material_gradient":
         acc = 8 if args.accumulator == "float64" else 4
         return cells * (2 * comps * hist_b + 12) + ext * 2 * 2 * acc

@@ -77,7 +77,7 @@ long long g_timing_count[kKCount];
 const char* const kKernelNames[kKCount] = {
     "coefficients", "fwd_h", "fwd_e", "sample", "quant_int8", "save_frame", "adj_receivers",
     "adj_source", "material_gradient", "adj_pml_e", "adj_e", "adj_pml_h", "adj_h", "tm_step_fwd",
-    "tm_step_adj", "tm_grad_reduce", "tm_src_reduce"};
+    "tm_step_adj", "tm_grad_reduce", "tm_src_reduce", "adj_weights_e", "adj_weights_h"};
 
 struct LaunchTimer {
     bool on;
@@ -1063,7 +1063,6 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
 
     // CPML weight scratch, shaped like the psi arrays
     DevBuf d_w;
-    Cpml we{}, wh{};
     Tm2dWeights tw{};
     if (pl.has_cpml && fused) {
         // the fused kernels keep the weights in LDS
@@ -1085,25 +1084,12 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             tw.wh[f] = p;
             p += across * g.pml[f];
         }
-    } else if (pl.has_cpml) {
-        long long tot = 0;
-        for (int f = 0; f < 6; ++f)
-            for (int w = 0; w < 2; ++w) tot += psi_elems(g, true, f, w) + psi_elems(g, false, f, w);
-        if (!d_w.alloc((size_t)tot * sizeof(float))) {
-            set_error("backward: out of device memory for CPML weights");
+    } else if (!tm) {
+        // six channel-weight arrays shared by the E and H halves of the adjoint
+        if (!d_w.alloc((size_t)6 * g.nshot * g.cells * sizeof(float))) {
+            set_error("backward: out of device memory for the adjoint channel weights (%lld bytes)",
+                      (long long)6 * g.nshot * g.cells * 4);
             return -1;
-        }
-        float* p = (float*)d_w.p;
-        for (int f = 0; f < 6; ++f)
-            for (int w = 0; w < 2; ++w) {
-                we.psi[f][w] = p;
-                p += psi_elems(g, true, f, w);
-                wh.psi[f][w] = p;
-                p += psi_elems(g, false, f, w);
-            }
-        for (int f = 0; f < 6; ++f) {
-            we.coef[f] = pl.cpe.coef[f];
-            wh.coef[f] = pl.cph.coef[f];
         }
     }
 
@@ -1139,8 +1125,6 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     }
     a.cpe = pl.cpe;
     a.cph = pl.cph;
-    a.we = we;
-    a.wh = wh;
     a.has_cpml = pl.has_cpml;
 
     MatGrad mg{};
@@ -1204,10 +1188,11 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             if (a.has_cpml) TIMED(tmr, kKAdjPmlH, launch_tm_adj_pml(false, tms.g, tms.f, tw, stream));
             TIMED(tmr, kKAdjH, launch_tm_adj_h(pl.order, tms.g, tms.f, tw, stream));
         } else {
-            if (a.has_cpml) TIMED(tmr, kKAdjPmlE, launch_adj_pml(true, g, a, stream));
-            TIMED(tmr, kKAdjE, launch_adj_e(pl.order, g, a, stream));
-            if (a.has_cpml) TIMED(tmr, kKAdjPmlH, launch_adj_pml(false, g, a, stream));
-            TIMED(tmr, kKAdjH, launch_adj_h(pl.order, g, a, stream));
+            float* W = (float*)d_w.p;
+            TIMED(tmr, kKAdjWeightsE, launch_adj_weights(true, g, a, W, stream));
+            TIMED(tmr, kKAdjE, launch_adj_gather_e(pl.order, g, a, W, stream));
+            TIMED(tmr, kKAdjWeightsH, launch_adj_weights(false, g, a, W, stream));
+            TIMED(tmr, kKAdjH, launch_adj_gather_h(pl.order, g, a, W, stream));
         }
     }
     if (tm && tms.unpack(P, stream)) {

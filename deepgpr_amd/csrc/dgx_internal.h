@@ -83,7 +83,6 @@ struct AdjArgs {
     F3 lE, lH;
     CF3 lEc, lHc;
     Cpml cpe, cph;  // lambda psi arrays + profiles
-    Cpml we, wh;    // CPML weight scratch shaped like the psi arrays
     int has_cpml;
 };
 
@@ -211,14 +210,16 @@ void launch_adj_source(const Grid& g, const int* loc, const float* cb, const flo
 void launch_material_gradient(const Grid& g, CF3 L, const MatGrad& mg, void* geps, void* gsig, bool fp64,
                               hipStream_t st);
 void launch_add_f64(const double* a, float* out, long long count, hipStream_t st);
-void launch_adj_pml(bool electric, const Grid& g, const AdjArgs& a, hipStream_t st);
-void launch_adj_e(int order, const Grid& g, const AdjArgs& a, hipStream_t st);
-void launch_adj_h(int order, const Grid& g, const AdjArgs& a, hipStream_t st);
+// two-pass adjoint: per-cell channel weights (6 x nshot x cells floats), then gathers
+void launch_adj_weights(bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st);
+void launch_adj_gather_e(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st);
+void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st);
 
 // Kernel ids of the optional per-launch timing (dgx_set_kernel_timing).
 enum KernelId : int {
     kKCoeff = 0, kKFwdH, kKFwdE, kKSample, kKQuant, kKSaveFrame, kKAdjRx, kKAdjSrc, kKMatGrad, kKAdjPmlE,
-    kKAdjE, kKAdjPmlH, kKAdjH, kKTmStepFwd, kKTmStepAdj, kKTmGradReduce, kKTmSrcReduce, kKCount
+    kKAdjE, kKAdjPmlH, kKAdjH, kKTmStepFwd, kKTmStepAdj, kKTmGradReduce, kKTmSrcReduce, kKAdjWeightsE,
+    kKAdjWeightsH, kKCount
 };
 
 }  // namespace dgx

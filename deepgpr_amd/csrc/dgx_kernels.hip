@@ -406,133 +406,17 @@ __global__ __launch_bounds__(256) void k_add_f64_to_f32(const double* __restrict
 // Adjoint CPML weights: (E-CPML)^T or (H-CPML)^T of one half step
 // (deepgpr_cpu.c:509-527, 1153-1239, 1332-1410)
 // ---------------------------------------------------------------------------
-// Each face cell advances its auxiliary cotangent and stores the weight its
-// CPML term contributes to the transposed derivative, in an array shaped like
-// the auxiliary array. The gather kernels read those weights.
+// Two-pass adjoint: every cell evaluates its six transposed-stencil channel
+// weights once (CPML weight, which also advances that cell's lambda psi, then
+// the curl term), and the gathers read them.
+// Channel (u, a) is stored at W + chan(u, a) * nshot * cells.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int chan_of(int u, int a) { return 2 * u + (a > u ? a - 1 : a); }
+
 template <bool ELECTRIC>
-__global__ __launch_bounds__(256) void k_adj_pml(Grid g, const float* __restrict__ upd, CF3 lam, Cpml cp,
-                                                 Cpml wout)
-{
-    const int f = blockIdx.y, s = blockIdx.z;
-    const int pp = g.pml[f];
-    if (pp <= 0) return;
-    const int axis = f >> 1;
-    // the two transverse axes in memory order (slow, fast) so that
-    // consecutive threads touch consecutive addresses
-    const int slow = axis == 0 ? 1 : 0, fast = axis == 2 ? 1 : 2;
-    const int n_face = pp;  // cells of the face region along its axis
-    const long long plane = (long long)g.N[slow] * g.N[fast];
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= plane * n_face) return;
-    const int rf_idx = (int)(idx / plane);
-    const long long rem = idx - (long long)rf_idx * plane;
-    const int cs = (int)(rem / g.N[fast]);
-    const int cf = (int)(rem - (long long)cs * g.N[fast]);
-    const int n_ax = g.N[axis];
-    int cax;
-    if ((f & 1) == 0) cax = ELECTRIC ? 1 + rf_idx : rf_idx;
-    else cax = n_ax - 1 - pp + rf_idx;
-    if (cax < 0 || cax >= n_ax) return;
-    int c[3];
-    c[axis] = cax;
-    c[slow] = cs;
-    c[fast] = cf;
-    const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
-    const int q = face_q(g, ELECTRIC, f, cax);
-    if (q < 0) return;
-    const long long m = (long long)c[0] * g.st0 + (long long)c[1] * g.N[2] + c[2];
-    const long long n = (long long)s * g.cells + m;
-    const float* R = cp.coef[f];
-    const float ra1 = R[q] - 1.0f, rb = R[pp + q], re = R[2 * pp + q], rf = R[3 * pp + q];
-    const float u = upd[m];
-    const float inv = 1.0f / g.d[axis];
-#pragma unroll
-    for (int w = 0; w < 2; ++w) {
-        int dims[3];
-        psi_dims(g, ELECTRIC, f, w, dims);
-        if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
-        const int t = face_target(axis, w);
-        const float sign = ELECTRIC ? e_face_sign(axis, w) : h_face_sign(axis, w);
-        const long long pi = psi_index(dims, s, c, axis, q);
-        const float lt = lam.c[t][n];
-        const float old = cp.psi[f][w][pi];
-        wout.psi[f][w][pi] = (sign * u * ra1 * lt - rf * old) * inv;
-        cp.psi[f][w][pi] = sign * u * rb * lt + re * old;
-    }
-}
-
-// Weight of transposed-stencil channel (src component u, axis a) at cell q.
-// ELECTRIC: channel of (E update + E-CPML)^T, built from lambda_E and ce_curl;
-// otherwise of (H update + H-CPML)^T from lambda_H and ch_curl.
-template <bool ELECTRIC>
-__device__ __forceinline__ float channel_weight(const Grid& g, const CF3& lam, const float* __restrict__ curl,
-                                                const Cpml& wts, bool cpml, int s, int u, int a, const int q[3])
-{
-    float acc = 0.0f;
-    const long long m = (long long)q[0] * g.st0 + (long long)q[1] * g.N[2] + q[2];
-    if (cpml) {
-        const int w = face_source(a, 0) == u ? 0 : 1;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const int f = 2 * a + side;
-            const int qq = face_q(g, ELECTRIC, f, q[a]);
-            if (qq < 0) continue;
-            int dims[3];
-            psi_dims(g, ELECTRIC, f, w, dims);
-            const int a1 = a == 2 ? 0 : a + 1, a2 = a == 0 ? 2 : a - 1;
-            if (q[a1] >= dims[a1] || q[a2] >= dims[a2]) continue;
-            acc += wts.psi[f][w][psi_index(dims, s, q, a, qq)];
-        }
-    }
-    const int t = 3 - u - a;  // component whose curl term differentiates u along a
-    const bool up = ELECTRIC ? e_updated(g, t, q) : h_updated(g, t, q);
-    if (up) {
-        const float k = axis_coeff(g, curl[m], a);
-        const int t1 = t == 2 ? 0 : t + 1;
-        // E: channel (t+2, t+1) gets +k, (t+1, t+2) gets -k; H the opposite.
-        const bool plus = (a == t1) == ELECTRIC;
-        const float lt = lam.c[t][(long long)s * g.cells + m];
-        acc += (plus ? k : -k) * lt;
-    }
-    return acc;
-}
-
-// (E-CPML + E update)^T: lambda_H += D-^T w(lambda_E). lambda_E is left
-// unscaled here; the ca scaling happens in k_adj_h.
-template <int ORDER>
-__global__ __launch_bounds__(256) void k_adj_e(Grid g, const float* __restrict__ ccurl, CF3 lamE, F3 lamH,
-                                               Cpml wts, int has_cpml)
-{
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.plane) return;
-    const int i = blockIdx.y, s = blockIdx.z;
-    const int j = p / g.N[2];
-    const int k = p - j * g.N[2];
-    const int c[3] = {i, j, k};
-    const long long n = (long long)s * g.cells + (long long)i * g.st0 + p;
-    const bool cpml = has_cpml != 0;
-#pragma unroll
-    for (int u = 2; u >= 0; --u) {
-        float acc = lamH.c[u][n];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (a == u) continue;
-            auto wf = [&](int off) {
-                int q[3] = {c[0], c[1], c[2]};
-                q[a] += off;
-                return channel_weight<true>(g, lamE, ccurl, wts, cpml, s, u, a, q);
-            };
-            acc += gather_bwd_T<ORDER>(wf, c[a], g.N[a]);
-        }
-        lamH.c[u][n] = acc;
-    }
-}
-
-// (H-CPML + H update)^T plus the deferred E scaling:
-// lambda_E = ca lambda_E (updated components) + D+^T w(lambda_H).
-template <int ORDER>
-__global__ __launch_bounds__(256) void k_adj_h(Grid g, const float* __restrict__ ca, const float* __restrict__ hcurl,
-                                               CF3 lamH, F3 lamE, Cpml wts, int has_cpml)
+__global__ __launch_bounds__(256) void k_adj_weights(Grid g, const float* __restrict__ curl,
+                                                     const float* __restrict__ upd, CF3 lam, Cpml cp,
+                                                     float* __restrict__ W, int has_cpml)
 {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.plane) return;
@@ -542,7 +426,96 @@ __global__ __launch_bounds__(256) void k_adj_h(Grid g, const float* __restrict__
     const int c[3] = {i, j, k};
     const long long m = (long long)i * g.st0 + p;
     const long long n = (long long)s * g.cells + m;
-    const bool cpml = has_cpml != 0;
+    const long long cstride = (long long)g.nshot * g.cells;
+    const bool cpml = has_cpml != 0 && in_any_face(g, ELECTRIC, c);
+    const float cu = curl[m];
+    const float lt3[3] = {lam.c[0][n], lam.c[1][n], lam.c[2][n]};
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a == u) continue;
+            const int t = 3 - u - a;
+            float acc = 0.0f;
+            if (cpml) {
+                const int w = face_source(a, 0) == u ? 0 : 1;
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    const int f = 2 * a + side;
+                    const int qq = face_q(g, ELECTRIC, f, c[a]);
+                    if (qq < 0) continue;
+                    int dims[3];
+                    psi_dims(g, ELECTRIC, f, w, dims);
+                    const int a1 = a == 2 ? 0 : a + 1, a2 = a == 0 ? 2 : a - 1;
+                    if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
+                    const float* R = cp.coef[f];
+                    const int pp = g.pml[f];
+                    const float ra1 = R[qq] - 1.0f, rb = R[pp + qq], re = R[2 * pp + qq], rf = R[3 * pp + qq];
+                    const float sign = ELECTRIC ? e_face_sign(a, w) : h_face_sign(a, w);
+                    const float uc = upd[m];
+                    const float inv = 1.0f / g.d[a];
+                    float* psi = cp.psi[f][w];
+                    const long long pi = psi_index(dims, s, c, a, qq);
+                    const float lt = lt3[face_target(a, w)];
+                    const float old = psi[pi];
+                    acc += (sign * uc * ra1 * lt - rf * old) * inv;
+                    psi[pi] = sign * uc * rb * lt + re * old;
+                }
+            }
+            const bool up = ELECTRIC ? e_updated(g, t, c) : h_updated(g, t, c);
+            if (up) {
+                const float kk = axis_coeff(g, cu, a);
+                const int t1 = t == 2 ? 0 : t + 1;
+                const bool plus = (a == t1) == ELECTRIC;
+                acc += (plus ? kk : -kk) * lt3[t];
+            }
+            W[chan_of(u, a) * cstride + n] = acc;
+        }
+    }
+}
+
+// lambda_H += D-^T W_E (the E-side weights); lambda_E is scaled later.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_adj_gather_e(Grid g, const float* __restrict__ W, F3 lamH)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.plane) return;
+    const int i = blockIdx.y, s = blockIdx.z;
+    const int j = p / g.N[2];
+    const int k = p - j * g.N[2];
+    const int c[3] = {i, j, k};
+    const long long n = (long long)s * g.cells + (long long)i * g.st0 + p;
+    const long long st[3] = {g.st0, (long long)g.N[2], 1};
+    const long long cstride = (long long)g.nshot * g.cells;
+#pragma unroll
+    for (int u = 2; u >= 0; --u) {
+        float acc = lamH.c[u][n];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a == u) continue;
+            const float* Wc = W + chan_of(u, a) * cstride + n;
+            auto wf = [&](int off) { return Wc[off * st[a]]; };
+            acc += gather_bwd_T<ORDER>(wf, c[a], g.N[a]);
+        }
+        lamH.c[u][n] = acc;
+    }
+}
+
+// lambda_E = ca lambda_E (updated components) + D+^T W_H.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_adj_gather_h(Grid g, const float* __restrict__ ca,
+                                                      const float* __restrict__ W, F3 lamE)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.plane) return;
+    const int i = blockIdx.y, s = blockIdx.z;
+    const int j = p / g.N[2];
+    const int k = p - j * g.N[2];
+    const int c[3] = {i, j, k};
+    const long long m = (long long)i * g.st0 + p;
+    const long long n = (long long)s * g.cells + m;
+    const long long st[3] = {g.st0, (long long)g.N[2], 1};
+    const long long cstride = (long long)g.nshot * g.cells;
     const float cav = ca[m];
 #pragma unroll
     for (int t = 2; t >= 0; --t) {
@@ -551,11 +524,8 @@ __global__ __launch_bounds__(256) void k_adj_h(Grid g, const float* __restrict__
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             if (a == t) continue;
-            auto wf = [&](int off) {
-                int q[3] = {c[0], c[1], c[2]};
-                q[a] += off;
-                return channel_weight<false>(g, lamH, hcurl, wts, cpml, s, t, a, q);
-            };
+            const float* Wc = W + chan_of(t, a) * cstride + n;
+            auto wf = [&](int off) { return Wc[off * st[a]]; };
             acc += gather_fwd_T<ORDER>(wf, c[a], g.N[a]);
         }
         lamE.c[t][n] = acc;
@@ -670,49 +640,31 @@ void launch_add_f64(const double* a, float* out, long long count, hipStream_t st
     hipLaunchKernelGGL(k_add_f64_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, a, out, count);
 }
 
-static long long max_face_cells(const Grid& g)
+void launch_adj_weights(bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st)
 {
-    long long best = 0;
-    for (int f = 0; f < 6; ++f) {
-        int axis = f >> 1;
-        int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
-        long long cells = (long long)g.pml[f] * g.N[a1] * g.N[a2];
-        if (cells > best) best = cells;
-    }
-    return best;
-}
-
-void launch_adj_pml(bool electric, const Grid& g, const AdjArgs& a, hipStream_t st)
-{
-    long long fc = max_face_cells(g);
-    if (!a.has_cpml || fc <= 0) return;
-    dim3 pgrid((unsigned)((fc + 255) / 256), 6, (unsigned)g.nshot);
+    const dim3 grid = cell_grid(g);
     if (electric)
-        hipLaunchKernelGGL(k_adj_pml<true>, pgrid, dim3(256), 0, st, g, a.cb, a.lEc, a.cpe, a.we);
+        hipLaunchKernelGGL(k_adj_weights<true>, grid, dim3(256), 0, st, g, a.ccurl, a.cb, a.lEc, a.cpe, W,
+                           a.has_cpml);
     else
-        hipLaunchKernelGGL(k_adj_pml<false>, pgrid, dim3(256), 0, st, g, a.hrhs, a.lHc, a.cph, a.wh);
+        hipLaunchKernelGGL(k_adj_weights<false>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.lHc, a.cph, W,
+                           a.has_cpml);
 }
 
-void launch_adj_e(int order, const Grid& g, const AdjArgs& a, hipStream_t st)
+void launch_adj_gather_e(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st)
 {
-    dim3 grid = cell_grid(g);
-    if (order == 8)
-        hipLaunchKernelGGL(k_adj_e<8>, grid, dim3(256), 0, st, g, a.ccurl, a.lEc, a.lH, a.we, a.has_cpml);
-    else if (order == 4)
-        hipLaunchKernelGGL(k_adj_e<4>, grid, dim3(256), 0, st, g, a.ccurl, a.lEc, a.lH, a.we, a.has_cpml);
-    else
-        hipLaunchKernelGGL(k_adj_e<2>, grid, dim3(256), 0, st, g, a.ccurl, a.lEc, a.lH, a.we, a.has_cpml);
+    const dim3 grid = cell_grid(g);
+    if (order == 8) hipLaunchKernelGGL(k_adj_gather_e<8>, grid, dim3(256), 0, st, g, W, a.lH);
+    else if (order == 4) hipLaunchKernelGGL(k_adj_gather_e<4>, grid, dim3(256), 0, st, g, W, a.lH);
+    else hipLaunchKernelGGL(k_adj_gather_e<2>, grid, dim3(256), 0, st, g, W, a.lH);
 }
 
-void launch_adj_h(int order, const Grid& g, const AdjArgs& a, hipStream_t st)
+void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st)
 {
-    dim3 grid = cell_grid(g);
-    if (order == 8)
-        hipLaunchKernelGGL(k_adj_h<8>, grid, dim3(256), 0, st, g, a.ca, a.hcurl, a.lHc, a.lE, a.wh, a.has_cpml);
-    else if (order == 4)
-        hipLaunchKernelGGL(k_adj_h<4>, grid, dim3(256), 0, st, g, a.ca, a.hcurl, a.lHc, a.lE, a.wh, a.has_cpml);
-    else
-        hipLaunchKernelGGL(k_adj_h<2>, grid, dim3(256), 0, st, g, a.ca, a.hcurl, a.lHc, a.lE, a.wh, a.has_cpml);
+    const dim3 grid = cell_grid(g);
+    if (order == 8) hipLaunchKernelGGL(k_adj_gather_h<8>, grid, dim3(256), 0, st, g, a.ca, W, a.lE);
+    else if (order == 4) hipLaunchKernelGGL(k_adj_gather_h<4>, grid, dim3(256), 0, st, g, a.ca, W, a.lE);
+    else hipLaunchKernelGGL(k_adj_gather_h<2>, grid, dim3(256), 0, st, g, a.ca, W, a.lE);
 }
 
 }  // namespace dgx
