@@ -4,6 +4,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -44,9 +46,11 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf()
+    ~DevBuf() { reset(); }
+    void reset()
     {
         if (p) (void)hipFreeAsync(p, st);
+        p = nullptr;
     }
     bool alloc(size_t bytes)
     {
@@ -572,6 +576,96 @@ void tm_source_lists(const Tm2dGrid& g, const std::vector<int>& sloc, int nsrc, 
     for (size_t b = 1; b < off.size(); ++b) off[b] += off[b - 1];
 }
 
+// Receivers of the fused TM kernels grouped by owning (tile, shot): (cell
+// index inside the tile, receiver index).
+void tm_receiver_lists(const Tm2dGrid& g, const std::vector<int>& rloc, int nrx, std::vector<int>& off,
+                       std::vector<int2>& ent)
+{
+    std::vector<std::pair<long long, int2>> tmp;
+    for (int s = 0; s < g.nshot; ++s)
+        for (int r = 0; r < nrx; ++r) {
+            const int* L = &rloc[((size_t)s * nrx + r) * 3];
+            tmp.push_back({(long long)tm_tile_of(g, L[0], L[1]) + (long long)g.ftiles * s,
+                           make_int2(tm_local(L[0], L[1]), r)});
+        }
+    std::stable_sort(tmp.begin(), tmp.end(), [](const std::pair<long long, int2>& x,
+                                                const std::pair<long long, int2>& y) { return x.first < y.first; });
+    off.assign((size_t)g.ftiles * g.nshot + 1, 0);
+    ent.clear();
+    for (const auto& e : tmp) {
+        off[(size_t)e.first + 1] += 1;
+        ent.push_back(e.second);
+    }
+    for (size_t b = 1; b < off.size(); ++b) off[b] += off[b - 1];
+}
+
+// Receiver groups (one per shot and cell, members in receiver order; goff /
+// gshot / gcell as built by the backward driver, cells in the compact plane)
+// listed for every (tile, shot) whose staged region (tile plus halo) holds the
+// cell: (staged index, first member, member count, 0).
+void tm_receiver_groups(const Tm2dGrid& g, int order, const std::vector<int>& goff, const std::vector<int>& gshot,
+                        const std::vector<long long>& gcell, std::vector<int>& off, std::vector<int4>& grp)
+{
+    const int he = tm_halo(order);
+    const int lc = kTmBJ + 2 * he;
+    const int nti = (g.NX + kTmBI - 1) / kTmBI;
+    std::vector<std::pair<long long, int4>> tmp;
+    for (size_t q = 0; q < gshot.size(); ++q) {
+        const int i = (int)(gcell[q] / g.NY), j = (int)(gcell[q] % g.NY);
+        for (int ti = std::max(0, (i - he) / kTmBI - 1); ti <= std::min(nti - 1, (i + he) / kTmBI); ++ti) {
+            const int r = i - (ti * kTmBI - he);
+            if (r < 0 || r >= kTmBI + 2 * he) continue;
+            for (int tj = std::max(0, (j - he) / kTmBJ - 1); tj <= std::min(g.ntj - 1, (j + he) / kTmBJ); ++tj) {
+                const int c = j - (tj * kTmBJ - he);
+                if (c < 0 || c >= lc) continue;
+                tmp.push_back({(long long)(ti * g.ntj + tj) + (long long)g.ftiles * gshot[q],
+                               make_int4(r * lc + c, goff[q], goff[q + 1] - goff[q], 0)});
+            }
+        }
+    }
+    std::stable_sort(tmp.begin(), tmp.end(), [](const std::pair<long long, int4>& x,
+                                                const std::pair<long long, int4>& y) { return x.first < y.first; });
+    off.assign((size_t)g.ftiles * g.nshot + 1, 0);
+    grp.clear();
+    for (const auto& e : tmp) {
+        off[(size_t)e.first + 1] += 1;
+        grp.push_back(e.second);
+    }
+    for (size_t k = 1; k < off.size(); ++k) off[k] += off[k - 1];
+}
+
+// Receiver groups of one call: one per (shot, cell), members in receiver
+// order; cell = linear index with strides (st0, st1, 1).
+void receiver_groups(const std::vector<int>& rloc, int nshot, int nrx, long long st0, long long st1,
+                     std::vector<int>& goff, std::vector<int>& gshot, std::vector<long long>& gcell,
+                     std::vector<int>& gmem)
+{
+    goff.assign(1, 0);
+    gshot.clear();
+    gcell.clear();
+    gmem.clear();
+    for (int s = 0; s < nshot; ++s) {
+        std::vector<std::pair<long long, int>> cells;
+        for (int r = 0; r < nrx; ++r) {
+            const int* L = &rloc[((size_t)s * nrx + r) * 3];
+            cells.push_back({(long long)L[0] * st0 + (long long)L[1] * st1 + L[2], r});
+        }
+        std::stable_sort(cells.begin(), cells.end(),
+                         [](const std::pair<long long, int>& x, const std::pair<long long, int>& y) {
+                             return x.first < y.first;
+                         });
+        for (size_t q = 0; q < cells.size(); ++q) {
+            if (q == 0 || cells[q].first != cells[q - 1].first) {
+                if (q) goff.push_back((int)gmem.size());
+                gshot.push_back(s);
+                gcell.push_back(cells[q].first);
+            }
+            gmem.push_back(cells[q].second);
+        }
+        if (!cells.empty()) goff.push_back((int)gmem.size());
+    }
+}
+
 // Device copy of a host-built list (nothing to do for an empty one).
 template <typename T>
 bool upload(DevBuf& d, const std::vector<T>& v, hipStream_t st)
@@ -691,24 +785,9 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     DevBuf d_roff, d_rent;
     TmReceiversFwd trx{};
     if (fused_rx) {
-        std::vector<std::pair<long long, int2>> tmp;
-        for (int s = 0; s < g.nshot; ++s)
-            for (int r = 0; r < P->nreceiver; ++r) {
-                const int* L = &hrloc[((size_t)s * P->nreceiver + r) * 3];
-                tmp.push_back({(long long)tm_tile_of(tg, L[0], L[1]) + (long long)tg.ftiles * s,
-                               make_int2(tm_local(L[0], L[1]), r)});
-            }
-        std::stable_sort(tmp.begin(), tmp.end(),
-                         [](const std::pair<long long, int2>& x, const std::pair<long long, int2>& y) {
-                             return x.first < y.first;
-                         });
-        std::vector<int> roff((size_t)tg.ftiles * g.nshot + 1, 0);
+        std::vector<int> roff;
         std::vector<int2> rent;
-        for (const auto& e : tmp) {
-            roff[(size_t)e.first + 1] += 1;
-            rent.push_back(e.second);
-        }
-        for (size_t b = 1; b < roff.size(); ++b) roff[b] += roff[b - 1];
+        tm_receiver_lists(tg, hrloc, P->nreceiver, roff, rent);
         if (!upload(d_roff, roff, stream) || !upload(d_rent, rent, stream)) {
             set_error("forward: out of device memory for receiver lists");
             return -1;
@@ -948,28 +1027,7 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
                         (!tm || P->receiver_component == 2);
     std::vector<int> goff(1, 0), gshot, gmem;
     std::vector<long long> gcell;
-    if (adj_rx) {
-        for (int s = 0; s < g.nshot; ++s) {
-            std::vector<std::pair<long long, int>> cells;
-            for (int r = 0; r < P->nreceiver; ++r) {
-                const int* L = &hrloc[((size_t)s * P->nreceiver + r) * 3];
-                cells.push_back({(long long)L[0] * gc.st0 + (long long)L[1] * gc.N[2] + L[2], r});
-            }
-            std::stable_sort(cells.begin(), cells.end(),
-                             [](const std::pair<long long, int>& x, const std::pair<long long, int>& y) {
-                                 return x.first < y.first;
-                             });
-            for (size_t q = 0; q < cells.size(); ++q) {
-                if (q == 0 || cells[q].first != cells[q - 1].first) {
-                    if (q) goff.push_back((int)gmem.size());
-                    gshot.push_back(s);
-                    gcell.push_back(cells[q].first);
-                }
-                gmem.push_back(cells[q].second);
-            }
-            if (!cells.empty()) goff.push_back((int)gmem.size());
-        }
-    }
+    if (adj_rx) receiver_groups(hrloc, g.nshot, P->nreceiver, gc.st0, gc.N[2], goff, gshot, gcell, gmem);
     DevBuf d_goff, d_gshot, d_gcell, d_gmem;
     ReceiverGroups rg{};
     if (adj_rx && !gshot.empty()) {
@@ -994,35 +1052,9 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     DevBuf d_xoff, d_xgrp;
     TmReceiversAdj tax{};
     if (fused && adj_rx && !gshot.empty()) {
-        const int he = tm_halo(pl.order);
-        const int lc = kTmBJ + 2 * he;
-        const int nti = (tg.NX + kTmBI - 1) / kTmBI;
-        std::vector<std::pair<long long, int4>> tmp;
-        for (size_t q = 0; q < gshot.size(); ++q) {
-            const int i = (int)(gcell[q] / tg.NY), j = (int)(gcell[q] % tg.NY);
-            for (int ti = std::max(0, (i - he) / kTmBI - 1); ti <= std::min(nti - 1, (i + he) / kTmBI); ++ti) {
-                const int r = i - (ti * kTmBI - he);
-                if (r < 0 || r >= kTmBI + 2 * he) continue;
-                for (int tj = std::max(0, (j - he) / kTmBJ - 1); tj <= std::min(tg.ntj - 1, (j + he) / kTmBJ);
-                     ++tj) {
-                    const int c = j - (tj * kTmBJ - he);
-                    if (c < 0 || c >= lc) continue;
-                    tmp.push_back({(long long)(ti * tg.ntj + tj) + (long long)tg.ftiles * gshot[q],
-                                   make_int4(r * lc + c, goff[q], goff[q + 1] - goff[q], 0)});
-                }
-            }
-        }
-        std::stable_sort(tmp.begin(), tmp.end(),
-                         [](const std::pair<long long, int4>& x, const std::pair<long long, int4>& y) {
-                             return x.first < y.first;
-                         });
-        std::vector<int> xoff((size_t)tg.ftiles * g.nshot + 1, 0);
+        std::vector<int> xoff;
         std::vector<int4> xgrp;
-        for (const auto& e : tmp) {
-            xoff[(size_t)e.first + 1] += 1;
-            xgrp.push_back(e.second);
-        }
-        for (size_t k = 1; k < xoff.size(); ++k) xoff[k] += xoff[k - 1];
+        tm_receiver_groups(tg, pl.order, goff, gshot, gcell, xoff, xgrp);
         if (!upload(d_xoff, xoff, stream) || !upload(d_xgrp, xgrp, stream)) {
             set_error("backward: out of device memory for receiver lists");
             return -1;
@@ -1261,6 +1293,286 @@ DGX_API int dgx_abi_version(void) { return DGX_ABI_VERSION; }
 
 DGX_API const char* dgx_last_error(void) { return g_error; }
 
+// ---------------------------------------------------------------------------
+// Batched 2D TM solves: instances of different sizes, orders and lengths
+// advance together, one launch per (order, phase) per step.
+// ---------------------------------------------------------------------------
+struct BatchInstance {
+    int id = 0;
+    int order = 2;
+    bool gradient = false;
+    TmBatchInst d{};
+    Tm2dState tms;
+    DevBuf l_soff, l_sent, l_roff, l_rent, l_xoff, l_xgrp, l_mem;
+    DevBuf dgrad, acc, terms;
+    float* grad_eps = nullptr;
+    float* grad_sig = nullptr;
+    long long ngrad = 0;
+    const float* rx_out = nullptr;
+    long long rx_count = 0;
+};
+
+struct Batch {
+    hipStream_t stream = nullptr;
+    int gstep = 0;
+    int next_id = 1;
+    std::vector<std::unique_ptr<BatchInstance>> active;
+    std::vector<std::pair<int, hipEvent_t>> finished;
+    bool dirty = true;
+    DevBuf desc[6], prefix[6];
+    size_t desc_cap[6] = {}, prefix_cap[6] = {};
+    int ninst[6] = {}, total[6] = {}, order_of[6] = {2, 4, 8, 2, 4, 8};
+};
+
+int order_slot(int order) { return order == 8 ? 2 : (order == 4 ? 1 : 0); }
+
+int batch_add(Batch* b, const dgx_solve* P, const dgx_solve* G)
+{
+    CallStream call(b->stream);
+    hipStream_t stream = b->stream;
+    Plan pl;
+    if (make_plan(P, false, pl)) return -1;
+    const Grid& g = pl.g;
+    if (P->nsource <= 0 || P->nreceiver <= 0 || !P->source_location || !P->receiver_location) {
+        set_error("batch: an instance needs sources and receivers");
+        return -1;
+    }
+    std::vector<int> hsloc((size_t)g.nshot * P->nsource * 3), hrloc((size_t)g.nshot * P->nreceiver * 3);
+    DGX_TRY(hipMemcpyAsync(hsloc.data(), P->source_location, hsloc.size() * 4, hipMemcpyDeviceToHost, stream),
+            "batch: copy source locations");
+    DGX_TRY(hipMemcpyAsync(hrloc.data(), P->receiver_location, hrloc.size() * 4, hipMemcpyDeviceToHost, stream),
+            "batch: copy receiver locations");
+    DGX_TRY(hipStreamSynchronize(stream), "batch: stream synchronize");
+    for (size_t q = 0; q < hsloc.size(); q += 3)
+        if (!loc_ok(g, &hsloc[q])) {
+            set_error("batch: source location outside the field grid");
+            return -1;
+        }
+    for (size_t q = 0; q < hrloc.size(); q += 3)
+        if (!loc_ok(g, &hrloc[q])) {
+            set_error("batch: receiver location outside the field grid");
+            return -1;
+        }
+    if (!tm2d_applies(P, pl, hsloc, hrloc) || P->receiver_component != 2) {
+        set_error("batch: only 2D Ez-TM solves (SOLVER_TM2D, Ez source and receivers on k = 0) can be batched");
+        return -1;
+    }
+    const bool gradient = G != nullptr;
+    if (gradient && (!pl.history || !pl.e_only || P->sampling_interval != 1 || !P->E_saved || !P->R_saved)) {
+        set_error("batch: gradient instances need an E-only history at sampling interval 1");
+        return -1;
+    }
+    auto inst = std::make_unique<BatchInstance>();
+    BatchInstance& I = *inst;
+    I.id = b->next_id++;
+    I.order = pl.order;
+    I.gradient = gradient;
+    launch_coefficients(g, P->eps_r_pad, P->sigma_pad, P->mu_r_pad, P->update_coeff, P->dt, stream);
+    if (I.tms.setup(pl, P, stream, true)) {
+        set_error("batch: out of device memory for an instance");
+        return -1;
+    }
+    const Tm2dGrid& tg = I.tms.g;
+    TmBatchInst& d = I.d;
+    d.g = tg;
+    d.f = I.tms.f;
+    d.pp = I.tms.pp;
+    std::vector<int> soff, roff;
+    std::vector<int2> sent, rent;
+    tm_source_lists(tg, hsloc, P->nsource, soff, sent);
+    tm_receiver_lists(tg, hrloc, P->nreceiver, roff, rent);
+    if (!upload(I.l_soff, soff, stream) || !upload(I.l_sent, sent, stream) || !upload(I.l_roff, roff, stream) ||
+        !upload(I.l_rent, rent, stream)) {
+        set_error("batch: out of device memory for lists");
+        return -1;
+    }
+    d.src.block_off = (const int*)I.l_soff.p;
+    d.src.entries = (const int2*)I.l_sent.p;
+    d.src.count = (int)sent.size();
+    d.src.wave = P->source_waveform;
+    d.src.nt = P->nt;
+    d.src.comp = 2;
+    d.src.dl = g.d[2];
+    d.src.vol = g.d[0] * g.d[1] * g.d[2];
+    d.rx.off = (const int*)I.l_roff.p;
+    d.rx.ent = (const int2*)I.l_rent.p;
+    d.rx.out = P->receiver_data;
+    d.rx.nrx = P->nreceiver;
+    d.rx.nt = P->nt;
+    d.rx.count = (int)rent.size();
+    d.mode = pl.comps == 3 ? 3 : 2;
+    d.box = pl.box;
+    d.nt = P->nt;
+    d.dt = P->dt;
+    I.rx_out = P->receiver_data;
+    I.rx_count = (long long)g.nshot * P->nt * P->nreceiver;
+    if (gradient) {
+        d.e_saved = P->E_saved;
+        d.e_final = P->R_saved;
+        d.snap = pl.snap;
+        d.cstride = pl.cstride;
+        d.kind = pl.kind;
+        d.nt_saved = pl.nt_saved;
+        d.sig_full = P->sigma_pad;
+        d.want_eps = G->eps_r_requires_grad == 1;
+        d.want_sig = G->sigma_requires_grad == 1;
+        I.grad_eps = G->grad_eps_r;
+        I.grad_sig = G->grad_sigma;
+        if ((d.want_eps && !I.grad_eps) || (d.want_sig && !I.grad_sig)) {
+            set_error("batch: null gradient output");
+            return -1;
+        }
+        std::vector<int> goff, gshot, gmem, xoff;
+        std::vector<long long> gcell;
+        std::vector<int4> xgrp;
+        receiver_groups(hrloc, g.nshot, P->nreceiver, g.N[1], 1, goff, gshot, gcell, gmem);
+        tm_receiver_groups(tg, pl.order, goff, gshot, gcell, xoff, xgrp);
+        if (!upload(I.l_xoff, xoff, stream) || !upload(I.l_xgrp, xgrp, stream) || !upload(I.l_mem, gmem, stream) ||
+            !I.dgrad.alloc((size_t)I.rx_count * sizeof(float))) {
+            set_error("batch: out of device memory for receiver groups");
+            return -1;
+        }
+        d.rxa.off = (const int*)I.l_xoff.p;
+        d.rxa.grp = (const int4*)I.l_xgrp.p;
+        d.rxa.member = (const int*)I.l_mem.p;
+        d.rxa.dgrad = (const float*)I.dgrad.p;
+        d.rxa.nrx = P->nreceiver;
+        d.rxa.nt = P->nt;
+        d.rxa.count = (int)xgrp.size();
+        I.ngrad = (long long)(g.N[0] - 1) * (g.N[1] - 1) * (g.N[2] - 1);
+        d.terms_len = 2 * (long long)g.nshot * pl.box.cells;
+        if (!I.acc.alloc((size_t)I.ngrad * 2 * sizeof(double)) ||
+            !I.terms.alloc((size_t)2 * d.terms_len * sizeof(float))) {
+            set_error("batch: out of device memory for gradient accumulators");
+            return -1;
+        }
+        DGX_TRY(hipMemsetAsync(I.acc.p, 0, (size_t)I.ngrad * 2 * sizeof(double), stream), "batch: zero accumulators");
+        d.geps = (double*)I.acc.p;
+        d.gsig = (double*)I.acc.p + I.ngrad;
+        d.terms = (float*)I.terms.p;
+    }
+    d.phase = 0;
+    d.start = b->gstep;
+    b->active.push_back(std::move(inst));
+    b->dirty = true;
+    DGX_TRY(hipGetLastError(), "batch: kernel launch");
+    return b->active.back()->id;
+}
+
+int batch_rebuild(Batch* b)
+{
+    for (int slot = 0; slot < 6; ++slot) {
+        std::vector<TmBatchInst> desc;
+        std::vector<int> prefix(1, 0);
+        for (auto& ip : b->active) {
+            const BatchInstance& I = *ip;
+            if (order_slot(I.order) + 3 * I.d.phase != slot) continue;
+            desc.push_back(I.d);
+            prefix.push_back(prefix.back() + I.d.g.ftiles * I.d.g.nshot);
+        }
+        b->ninst[slot] = (int)desc.size();
+        b->total[slot] = prefix.back();
+        if (desc.empty()) continue;
+        const size_t db = desc.size() * sizeof(TmBatchInst), pb = prefix.size() * sizeof(int);
+        if (db > b->desc_cap[slot]) {
+            b->desc[slot].reset();
+            if (!b->desc[slot].alloc(db)) return -1;
+            b->desc_cap[slot] = db;
+        }
+        if (pb > b->prefix_cap[slot]) {
+            b->prefix[slot].reset();
+            if (!b->prefix[slot].alloc(pb)) return -1;
+            b->prefix_cap[slot] = pb;
+        }
+        if (hipMemcpyAsync(b->desc[slot].p, desc.data(), db, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+            hipMemcpyAsync(b->prefix[slot].p, prefix.data(), pb, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+            return -1;
+    }
+    b->dirty = false;
+    return 0;
+}
+
+// An instance whose phase ended at the current global step: start its
+// adjoint, or finish it.
+int batch_phase_end(Batch* b, BatchInstance& I, bool& finished)
+{
+    hipStream_t stream = b->stream;
+    TmBatchInst& d = I.d;
+    finished = false;
+    if (d.phase == 0 && I.gradient) {
+        const long long field = (long long)d.g.nshot * d.g.plane;
+        const float* ez = (d.nt & 1) ? d.pp.Ez[1] : d.pp.Ez[0];
+        HistWrite hw{};
+        hw.mode = d.mode;
+        hw.box = d.box;
+        hw.e_kind = d.kind;
+        hw.e_base = d.e_final;
+        hw.e_cstride = d.snap;
+        launch_tm_final_frame(d.g, ez, hw, stream);
+        launch_scale(I.rx_out, (float*)I.dgrad.p, 2.0f, I.rx_count, stream);
+        // the adjoint starts from zero cotangents in both ping-pong copies
+        for (int k = 0; k < 2; ++k) {
+            DGX_TRY(hipMemsetAsync(d.pp.Ez[k], 0, field * 4, stream), "batch: clear");
+            DGX_TRY(hipMemsetAsync(d.pp.Hx[k], 0, field * 4, stream), "batch: clear");
+            DGX_TRY(hipMemsetAsync(d.pp.Hy[k], 0, field * 4, stream), "batch: clear");
+            for (int f4 = 0; f4 < 4; ++f4) {
+                if (I.tms.psi_len_e[f4])
+                    DGX_TRY(hipMemsetAsync(d.pp.psi_e[k][f4], 0, I.tms.psi_len_e[f4] * 4, stream), "batch: clear");
+                if (I.tms.psi_len_h[f4])
+                    DGX_TRY(hipMemsetAsync(d.pp.psi_h[k][f4], 0, I.tms.psi_len_h[f4] * 4, stream), "batch: clear");
+            }
+        }
+        d.phase = 1;
+        d.start = b->gstep;
+        return 0;
+    }
+    if (d.phase == 1) {
+        // the last saved step (it = 0) has no later launch to sum it
+        const float* last = d.terms + (size_t)((d.nt - 1) & 1) * d.terms_len;
+        launch_tm_grad_reduce(d.g, d.box, last, d.want_eps, d.want_sig, d.geps, d.gsig, true, stream);
+        if (d.want_eps) launch_add_f64(d.geps, I.grad_eps, I.ngrad, stream);
+        if (d.want_sig) launch_add_f64(d.gsig, I.grad_sig, I.ngrad, stream);
+    }
+    finished = true;
+    return 0;
+}
+
+int batch_step(Batch* b, int steps)
+{
+    CallStream call(b->stream);
+    for (int k = 0; k < steps && !b->active.empty(); ++k) {
+        if (b->dirty && batch_rebuild(b)) {
+            set_error("batch: out of device memory for descriptors");
+            return -1;
+        }
+        for (int slot = 0; slot < 6; ++slot)
+            launch_tm_batch(b->order_of[slot], slot >= 3, (const TmBatchInst*)b->desc[slot].p,
+                            (const int*)b->prefix[slot].p, b->ninst[slot], b->total[slot], b->gstep, b->stream);
+        b->gstep += 1;
+        for (size_t q = 0; q < b->active.size();) {
+            BatchInstance& I = *b->active[q];
+            if (b->gstep - I.d.start < I.d.nt) {
+                ++q;
+                continue;
+            }
+            bool finished = false;
+            if (batch_phase_end(b, I, finished)) return -1;
+            b->dirty = true;
+            if (!finished) {
+                ++q;
+                continue;
+            }
+            hipEvent_t ev;
+            DGX_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "batch: event");
+            DGX_TRY(hipEventRecord(ev, b->stream), "batch: event");
+            b->finished.push_back({I.id, ev});
+            b->active.erase(b->active.begin() + (long)q);  // scratch is freed stream-ordered
+        }
+    }
+    DGX_TRY(hipGetLastError(), "batch: kernel launch");
+    return (int)b->active.size();
+}
+
 DGX_API int dgx_forward(const dgx_solve* solve, dgx_stream_t stream)
 {
     g_error[0] = '\0';
@@ -1271,6 +1583,76 @@ DGX_API int dgx_backward(const dgx_solve* solve, dgx_stream_t stream)
 {
     g_error[0] = '\0';
     return run_backward(solve, (hipStream_t)stream);
+}
+
+DGX_API void* dgx_batch_create(dgx_stream_t stream)
+{
+    g_error[0] = '\0';
+    Batch* b = new (std::nothrow) Batch();
+    if (!b) {
+        set_error("batch: out of host memory");
+        return nullptr;
+    }
+    b->stream = (hipStream_t)stream;
+    return b;
+}
+
+DGX_API int dgx_batch_add(void* batch, const dgx_solve* forward, const dgx_solve* gradient)
+{
+    g_error[0] = '\0';
+    if (!batch || !forward) {
+        set_error("batch: null argument");
+        return -1;
+    }
+    return batch_add((Batch*)batch, forward, gradient);
+}
+
+DGX_API int dgx_batch_step(void* batch, int steps)
+{
+    g_error[0] = '\0';
+    if (!batch) {
+        set_error("batch: null argument");
+        return -1;
+    }
+    return batch_step((Batch*)batch, steps);
+}
+
+DGX_API int dgx_batch_poll(void* batch, int id)
+{
+    g_error[0] = '\0';
+    Batch* b = (Batch*)batch;
+    if (!b) return -1;
+    for (size_t q = 0; q < b->finished.size(); ++q) {
+        if (b->finished[q].first != id) continue;
+        const hipError_t e = hipEventQuery(b->finished[q].second);
+        if (e == hipErrorNotReady) return 0;
+        (void)hipEventDestroy(b->finished[q].second);
+        b->finished.erase(b->finished.begin() + (long)q);
+        if (e != hipSuccess) {
+            set_error("batch: %s", hipGetErrorString(e));
+            return -1;
+        }
+        return 1;
+    }
+    for (auto& ip : b->active)
+        if (ip->id == id) return 0;
+    set_error("batch: unknown or already collected instance %d", id);
+    return -1;
+}
+
+DGX_API int dgx_batch_active(void* batch) { return batch ? (int)((Batch*)batch)->active.size() : 0; }
+
+DGX_API void dgx_batch_destroy(void* batch)
+{
+    Batch* b = (Batch*)batch;
+    if (!b) return;
+    (void)hipStreamSynchronize(b->stream);
+    for (auto& f : b->finished) (void)hipEventDestroy(f.second);
+    {
+        CallStream call(b->stream);
+        b->active.clear();
+    }
+    delete b;
 }
 
 DGX_API int dgx_codec_roundtrip(const float* input, void* encoded, float* decoded, long long count,

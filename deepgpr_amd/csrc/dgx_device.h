@@ -133,6 +133,36 @@ __device__ __forceinline__ float dfwd(const float* __restrict__ f, IDX id, typen
     }
 }
 
+// The unclamped differences (the full-radius branch of dbwd / dfwd), for
+// cells at least R away from every face.
+template <int ORDER>
+__device__ __forceinline__ float dbwd_full(const float* __restrict__ f, long long id, long long st)
+{
+    if constexpr (ORDER <= 2) {
+        return f[id] - f[id - st];
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 1; r <= R; ++r) acc += stencil_c(R, r) * (f[id + (r - 1) * st] - f[id - r * st]);
+        return acc;
+    }
+}
+
+template <int ORDER>
+__device__ __forceinline__ float dfwd_full(const float* __restrict__ f, long long id, long long st)
+{
+    if constexpr (ORDER <= 2) {
+        return f[id + st] - f[id];
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 1; r <= R; ++r) acc += stencil_c(R, r) * (f[id + r * st] - f[id - (r - 1) * st]);
+        return acc;
+    }
+}
+
 // Transposes of the staggered differences (deepgpr_cpu.c:430-511).
 template <int ORDER, typename WF>
 __device__ __forceinline__ float gather_bwd_T(WF wf, int c, int n)
@@ -228,6 +258,14 @@ __device__ __forceinline__ bool h_updated(const Grid& g, int comp, const int c[3
     if (c[comp] <= 0 || c[comp] >= g.N[comp] - 1) return false;
     if (c[a1] >= g.N[a1] - 1 || c[a2] >= g.N[a2] - 1) return false;
     return true;
+}
+
+// A cell at least R away from every face: no difference is clamped there and
+// every E and H component is updated (an axis with N = 2 has no such cell).
+__device__ __forceinline__ bool plain_cell(const Grid& g, int R, const int c[3])
+{
+    return c[0] >= R && c[0] <= g.N[0] - 1 - R && c[1] >= R && c[1] <= g.N[1] - 1 - R && c[2] >= R &&
+           c[2] <= g.N[2] - 1 - R;
 }
 
 // Curl coefficient on axis a: c1 * dx / d_a unless d_a == dx (deepgpr_cpu.c:693-694).

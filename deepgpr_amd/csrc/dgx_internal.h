@@ -171,6 +171,42 @@ struct TmAdjGrad {
     MatGrad mg;
 };
 
+// One instance of a batched 2D TM launch (dgx_batch_*). Index 0 of the
+// ping-pong buffers holds the state at the start of the current phase; the
+// kernels derive the step, the buffer parity, the history frame and the
+// material-term buffers from the global step counter.
+struct TmBatchInst {
+    Tm2dGrid g;
+    Tm2dFields f;         // coefficient planes and CPML profiles (field pointers unused)
+    TmPingPong pp;
+    SourceBlocks src;     // forward sources
+    TmReceiversFwd rx;    // forward receivers
+    TmReceiversAdj rxa;   // adjoint receiver groups
+    void* e_saved;        // E^n frames (E-only history)
+    void* e_final;        // the final E frame
+    long long snap;       // elements of one frame of one component
+    long long cstride;    // elements between components of the history
+    int kind, mode, nt_saved;
+    Box box;
+    float* terms;         // two buffers of terms_len floats
+    long long terms_len;
+    double* geps;
+    double* gsig;
+    const float* sig_full;
+    float dt;
+    int want_eps, want_sig;
+    int nt;
+    int phase;            // 0 forward, 1 adjoint
+    int start;            // global step at which the phase started
+};
+
+// Batched step of every listed instance of one order and phase:
+// prefix[k] .. prefix[k+1] are the workgroups of instance k.
+void launch_tm_batch(int order, bool adjoint, const TmBatchInst* insts, const int* prefix, int ninst,
+                     int total_groups, int gstep, hipStream_t st);
+void launch_tm_final_frame(const Tm2dGrid& g, const float* ez, const HistWrite& hw, hipStream_t st);
+void launch_scale(const float* in, float* out, float factor, long long count, hipStream_t st);
+
 void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
                         const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
                         hipStream_t st);

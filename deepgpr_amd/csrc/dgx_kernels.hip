@@ -60,8 +60,33 @@ __global__ __launch_bounds__(256) void k_fwd_h(Grid g, const float* __restrict__
     const long long m = (long long)i * g.st0 + p;
     const long long n = (long long)s * g.cells + m;
     const long long st[3] = {g.st0, (long long)g.N[2], 1};
-    const bool up0 = h_updated(g, 0, c), up1 = h_updated(g, 1, c), up2 = h_updated(g, 2, c);
     const bool pml = has_cpml && in_any_face(g, false, c);
+    if (!pml && plain_cell(g, Radius<ORDER>::value, c)) {
+        // every component updated, no clamped difference, no CPML term
+        float h0 = H.c[0][n], h1 = H.c[1][n], h2 = H.c[2][n];
+        const float u1 = hcurl[m];
+        const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
+        {
+            const float dEz = dfwd_full<ORDER>(E.c[2], n, st[1]);
+            const float dEy = dfwd_full<ORDER>(E.c[1], n, 1);
+            h0 = h0 - uy * dEz + uz * dEy;
+        }
+        {
+            const float dEx = dfwd_full<ORDER>(E.c[0], n, 1);
+            const float dEz = dfwd_full<ORDER>(E.c[2], n, st[0]);
+            h1 = h1 - uz * dEx + u1 * dEz;
+        }
+        {
+            const float dEy = dfwd_full<ORDER>(E.c[1], n, st[0]);
+            const float dEx = dfwd_full<ORDER>(E.c[0], n, st[1]);
+            h2 = h2 - u1 * dEy + uy * dEx;
+        }
+        H.c[0][n] = h0;
+        H.c[1][n] = h1;
+        H.c[2][n] = h2;
+        return;
+    }
+    const bool up0 = h_updated(g, 0, c), up1 = h_updated(g, 1, c), up2 = h_updated(g, 2, c);
     if (!(up0 || up1 || up2 || pml)) return;
 
     float h[3] = {H.c[0][n], H.c[1][n], H.c[2][n]};
@@ -152,46 +177,65 @@ __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__
     const float eold[3] = {e[0], e[1], e[2]};
     const float u0 = ca[m], u1 = ccurl[m];
     const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
-    if (up[0]) {
-        float dHz = dbwd<ORDER>(Hf.c[2], n, st[1], j, g.N[1]);
-        float dHy = dbwd<ORDER>(Hf.c[1], n, 1, k, g.N[2]);
-        e[0] = u0 * e[0] + uy * dHz - uz * dHy;
-    }
-    if (up[1]) {
-        float dHx = dbwd<ORDER>(Hf.c[0], n, 1, k, g.N[2]);
-        float dHz = dbwd<ORDER>(Hf.c[2], n, st[0], i, g.N[0]);
-        e[1] = u0 * e[1] + uz * dHx - u1 * dHz;
-    }
-    if (up[2]) {
-        float dHy = dbwd<ORDER>(Hf.c[1], n, st[0], i, g.N[0]);
-        float dHx = dbwd<ORDER>(Hf.c[0], n, st[1], j, g.N[1]);
-        e[2] = u0 * e[2] + u1 * dHy - uy * dHx;
-    }
     const float cbv = cb[m];
-    if (pml) {
+    if (!pml && plain_cell(g, Radius<ORDER>::value, c)) {
+        // every component updated, no clamped difference, no CPML term
+        {
+            const float dHz = dbwd_full<ORDER>(Hf.c[2], n, st[1]);
+            const float dHy = dbwd_full<ORDER>(Hf.c[1], n, 1);
+            e[0] = u0 * e[0] + uy * dHz - uz * dHy;
+        }
+        {
+            const float dHx = dbwd_full<ORDER>(Hf.c[0], n, 1);
+            const float dHz = dbwd_full<ORDER>(Hf.c[2], n, st[0]);
+            e[1] = u0 * e[1] + uz * dHx - u1 * dHz;
+        }
+        {
+            const float dHy = dbwd_full<ORDER>(Hf.c[1], n, st[0]);
+            const float dHx = dbwd_full<ORDER>(Hf.c[0], n, st[1]);
+            e[2] = u0 * e[2] + u1 * dHy - uy * dHx;
+        }
+    } else {
+        if (up[0]) {
+            float dHz = dbwd<ORDER>(Hf.c[2], n, st[1], j, g.N[1]);
+            float dHy = dbwd<ORDER>(Hf.c[1], n, 1, k, g.N[2]);
+            e[0] = u0 * e[0] + uy * dHz - uz * dHy;
+        }
+        if (up[1]) {
+            float dHx = dbwd<ORDER>(Hf.c[0], n, 1, k, g.N[2]);
+            float dHz = dbwd<ORDER>(Hf.c[2], n, st[0], i, g.N[0]);
+            e[1] = u0 * e[1] + uz * dHx - u1 * dHz;
+        }
+        if (up[2]) {
+            float dHy = dbwd<ORDER>(Hf.c[1], n, st[0], i, g.N[0]);
+            float dHx = dbwd<ORDER>(Hf.c[0], n, st[1], j, g.N[1]);
+            e[2] = u0 * e[2] + u1 * dHy - uy * dHx;
+        }
+        if (pml) {
 #pragma unroll
-        for (int f = 0; f < 6; ++f) {
-            const int axis = f >> 1;
-            const int q = face_q(g, true, f, c[axis]);
-            if (q < 0) continue;
-            const float* R = cp.coef[f];
-            const int pp = g.pml[f];
-            const float ra1 = R[q] - 1.0f, rb = R[pp + q], re = R[2 * pp + q], rf = R[3 * pp + q];
+            for (int f = 0; f < 6; ++f) {
+                const int axis = f >> 1;
+                const int q = face_q(g, true, f, c[axis]);
+                if (q < 0) continue;
+                const float* R = cp.coef[f];
+                const int pp = g.pml[f];
+                const float ra1 = R[q] - 1.0f, rb = R[pp + q], re = R[2 * pp + q], rf = R[3 * pp + q];
 #pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                int dims[3];
-                psi_dims(g, true, f, w, dims);
-                const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
-                if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
-                const int t = face_target(axis, w), sc = face_source(axis, w);
-                const float der = dbwd<ORDER>(Hf.c[sc], n, st[axis], c[axis], g.N[axis]) / g.d[axis];
-                float* psi = cp.psi[f][w];
-                const long long pi = psi_index(dims, s, c, axis, q);
-                const float phi = psi[pi];
-                const float corr = cbv * (ra1 * der + rb * phi);
-                if (e_face_sign(axis, w) < 0.0f) e[t] -= corr;
-                else e[t] += corr;
-                psi[pi] = re * phi - rf * der;
+                for (int w = 0; w < 2; ++w) {
+                    int dims[3];
+                    psi_dims(g, true, f, w, dims);
+                    const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
+                    if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
+                    const int t = face_target(axis, w), sc = face_source(axis, w);
+                    const float der = dbwd<ORDER>(Hf.c[sc], n, st[axis], c[axis], g.N[axis]) / g.d[axis];
+                    float* psi = cp.psi[f][w];
+                    const long long pi = psi_index(dims, s, c, axis, q);
+                    const float phi = psi[pi];
+                    const float corr = cbv * (ra1 * der + rb * phi);
+                    if (e_face_sign(axis, w) < 0.0f) e[t] -= corr;
+                    else e[t] += corr;
+                    psi[pi] = re * phi - rf * der;
+                }
             }
         }
     }
@@ -430,6 +474,22 @@ __global__ __launch_bounds__(256) void k_adj_weights(Grid g, const float* __rest
     const bool cpml = has_cpml != 0 && in_any_face(g, ELECTRIC, c);
     const float cu = curl[m];
     const float lt3[3] = {lam.c[0][n], lam.c[1][n], lam.c[2][n]};
+    if (!cpml && plain_cell(g, 1, c)) {
+        // every component updated and no CPML term: the curl terms alone
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (a == u) continue;
+                const int t = 3 - u - a;
+                const float kk = axis_coeff(g, cu, a);
+                const int t1 = t == 2 ? 0 : t + 1;
+                const bool plus = (a == t1) == ELECTRIC;
+                W[chan_of(u, a) * cstride + n] = 0.0f + (plus ? kk : -kk) * lt3[t];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
 #pragma unroll

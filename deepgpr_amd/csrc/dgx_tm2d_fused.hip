@@ -104,8 +104,9 @@ __device__ __forceinline__ Cpml4 cpml_at(const float* R, int p, int q)
 // indices) is computed once.
 // ---------------------------------------------------------------------------
 template <int ORDER>
-__global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
-                                                     TmReceiversFwd rx, HistWrite hist, int it)
+__device__ __forceinline__ void tm_fwd_tile(const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                                            const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hist,
+                                            int it, int tile, int s)
 {
     using T = Tile<ORDER>;
     constexpr int R = T::R, HE = T::HE, LR = T::LR, LC = T::LC, HR = T::HR, HC = T::HC;
@@ -115,9 +116,6 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
     __shared__ float sHy[HR * BJ];
     __shared__ float sHx[BI * HC];
     __shared__ float sEn[BI * BJ];
-    const int tile = step_tile(g.ftiles);
-    if (tile < 0) return;
-    const int s = blockIdx.y;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int ti = tile / g.ntj, tj = tile - ti * g.ntj;
@@ -359,6 +357,15 @@ __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, T
     }
 }
 
+template <int ORDER>
+__global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
+                                                     TmReceiversFwd rx, HistWrite hist, int it)
+{
+    const int tile = step_tile(g.ftiles);
+    if (tile < 0) return;
+    tm_fwd_tile<ORDER>(g, f, pp, src, rx, hist, it, tile, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint: receivers^T, source and material terms, (E step)^T, (H step)^T
 // ---------------------------------------------------------------------------
@@ -395,8 +402,9 @@ __device__ __forceinline__ void reduce_terms(const Tm2dGrid& g, const Box& b, co
 }
 
 template <int ORDER, typename ACC>
-__global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
-                                                     TmReceiversAdj rx, TmAdjGrad ag, int it)
+__device__ __forceinline__ void tm_adj_tile(const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                                            const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag,
+                                            int it, int tile, int s)
 {
     using T = Tile<ORDER>;
     constexpr int R = T::R, HE = T::HE, LR = T::LR, LC = T::LC, HR = T::HR, HC = T::HC;
@@ -407,9 +415,6 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
     __shared__ float sB[BI * LC];   // (Hx, y) weights (row stride LC), then (Ez, y) weights (row stride HC)
     __shared__ float sHy[HR * BJ];  // new lambda_Hy on rows [i0-R, i0+BI+R-1)
     __shared__ float sHx[BI * HC];  // new lambda_Hx on columns [j0-R, j0+BJ+R-1)
-    const int tile = step_tile(g.ftiles);
-    if (tile < 0) return;
-    const int s = blockIdx.y;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int ti = tile / g.ntj, tj = tile - ti * g.ntj;
@@ -774,6 +779,15 @@ __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, T
     }
 }
 
+template <int ORDER, typename ACC>
+__global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
+                                                     TmReceiversAdj rx, TmAdjGrad ag, int it)
+{
+    const int tile = step_tile(g.ftiles);
+    if (tile < 0) return;
+    tm_adj_tile<ORDER, ACC>(g, f, pp, src, rx, ag, it, tile, blockIdx.y);
+}
+
 // Material gradient of one saved step: per-shot terms summed in shot order.
 template <typename ACC>
 __global__ __launch_bounds__(256) void k_tm_grad_reduce(Tm2dGrid g, Box b, const float* __restrict__ terms,
@@ -795,6 +809,183 @@ __global__ __launch_bounds__(256) void k_tm_src_reduce(const float* __restrict__
     float acc = gsrc[idx];
     for (int s = 0; s < nshot; ++s) acc += vals[((long long)it * nshot + s) * nsrc + k];
     gsrc[idx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Batched steps: one launch advances every instance of the batch by one step.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int batch_instance(const int* __restrict__ prefix, int ninst, int b)
+{
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int batch_group(int total)
+{
+    const int chunk = gridDim.x >> 3;
+    const int b = (blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+    return b < total ? b : -1;
+}
+
+__device__ __forceinline__ TmPingPong batch_buffers(const TmPingPong& pp, int parity)
+{
+    if (!parity) return pp;
+    TmPingPong q;
+    q.Ez[0] = pp.Ez[1];
+    q.Ez[1] = pp.Ez[0];
+    q.Hx[0] = pp.Hx[1];
+    q.Hx[1] = pp.Hx[0];
+    q.Hy[0] = pp.Hy[1];
+    q.Hy[1] = pp.Hy[0];
+#pragma unroll
+    for (int f4 = 0; f4 < 4; ++f4) {
+        q.psi_e[0][f4] = pp.psi_e[1][f4];
+        q.psi_e[1][f4] = pp.psi_e[0][f4];
+        q.psi_h[0][f4] = pp.psi_h[1][f4];
+        q.psi_h[1][f4] = pp.psi_h[0][f4];
+    }
+    return q;
+}
+
+__device__ __forceinline__ size_t hist_elem_bytes(int kind) { return kind == kF32 ? 4 : 2; }
+
+template <int ORDER>
+__global__ __launch_bounds__(NTF) void k_tm_batch_fwd(const TmBatchInst* __restrict__ insts,
+                                                      const int* __restrict__ prefix, int ninst, int total,
+                                                      int gstep)
+{
+    const int b = batch_group(total);
+    if (b < 0) return;
+    const int k = batch_instance(prefix, ninst, b);
+    const TmBatchInst& I = insts[k];
+    const Tm2dGrid g = I.g;
+    const Tm2dFields f = I.f;
+    const int local = b - prefix[k];
+    const int s = local / g.ftiles, tile = local - s * g.ftiles;
+    const int it = gstep - I.start;
+    const TmPingPong pp = batch_buffers(I.pp, it & 1);
+    HistWrite hw{};
+    hw.mode = I.mode;
+    hw.box = I.box;
+    if (I.e_saved) {  // E-only history at every step (sampling interval 1)
+        hw.active = 1;
+        hw.save_e = 1;
+        hw.e_kind = I.kind;
+        hw.e_base = (unsigned char*)I.e_saved + (size_t)it * I.snap * hist_elem_bytes(I.kind);
+        hw.e_cstride = I.cstride;
+    }
+    tm_fwd_tile<ORDER>(g, f, pp, I.src, I.rx, hw, it, tile, s);
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(NTA) void k_tm_batch_adj(const TmBatchInst* __restrict__ insts,
+                                                      const int* __restrict__ prefix, int ninst, int total,
+                                                      int gstep)
+{
+    const int b = batch_group(total);
+    if (b < 0) return;
+    const int k = batch_instance(prefix, ninst, b);
+    const TmBatchInst& I = insts[k];
+    const Tm2dGrid g = I.g;
+    const Tm2dFields f = I.f;
+    const int local = b - prefix[k];
+    const int s = local / g.ftiles, tile = local - s * g.ftiles;
+    const int done = gstep - I.start;
+    const int it = I.nt - 1 - done;
+    const TmPingPong pp = batch_buffers(I.pp, done & 1);
+    const size_t eb = hist_elem_bytes(I.kind);
+    TmAdjGrad ag{};
+    ag.geps = I.geps;
+    ag.gsig = I.gsig;
+    ag.ez_comp = I.mode == 3 ? 2 : 0;
+    MatGrad& mg = ag.mg;
+    mg.sig = I.sig_full;
+    mg.kind = I.kind;
+    mg.e_only = 1;
+    mg.mode = I.mode;
+    mg.weight = 1;
+    mg.want_eps = I.want_eps;
+    mg.want_sig = I.want_sig;
+    mg.dt = I.dt;
+    mg.box = I.box;
+    mg.eframe = (const unsigned char*)I.e_saved + (size_t)it * I.snap * eb;
+    mg.e_cstride = I.cstride;
+    if (it + 1 < I.nt_saved) {
+        mg.sframe = (const unsigned char*)I.e_saved + (size_t)(it + 1) * I.snap * eb;
+        mg.s_cstride = I.cstride;
+    } else {
+        mg.sframe = I.e_final;
+        mg.s_cstride = I.snap;
+    }
+    // sampling interval 1: buffer (nt - 1 - it) & 1 takes this step's terms,
+    // the other one holds the previous step's (summed by this launch)
+    if (I.want_eps || I.want_sig) {
+        ag.terms = I.terms + (size_t)(done & 1) * I.terms_len;
+        if (done > 0) ag.pending = I.terms + (size_t)((done - 1) & 1) * I.terms_len;
+    }
+    SourceBlocks none{};
+    tm_adj_tile<ORDER, double>(g, f, pp, none, I.rxa, ag, it, tile, s);
+}
+
+// The final E frame of an E-only history from the compact Ez (Ex, Ey are zero).
+__global__ __launch_bounds__(256) void k_tm_final_frame(Tm2dGrid g, const float* __restrict__ ez, HistWrite hw)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)g.nshot * hw.box.cells) return;
+    const int s = (int)(idx / hw.box.cells);
+    const long long h = idx - (long long)s * hw.box.cells;
+    const int i = (int)(h / hw.box.ny) + hw.box.x0, j = (int)(h % hw.box.ny) + hw.box.y0;
+    const float v = ez[(long long)s * g.plane + (long long)i * g.NY + j];
+    const int comps = hw.mode == 3 ? 3 : 1;
+    for (int ci = 0; ci < comps; ++ci)
+        store_hist(hw.e_base, ci * hw.e_cstride + idx, ci == comps - 1 ? v : 0.0f, hw.e_kind);
+}
+
+__global__ __launch_bounds__(256) void k_scale(const float* __restrict__ in, float* __restrict__ out, float a,
+                                               long long count)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < count) out[idx] = a * in[idx];
+}
+
+void launch_tm_batch(int order, bool adjoint, const TmBatchInst* insts, const int* prefix, int ninst,
+                     int total_groups, int gstep, hipStream_t st)
+{
+    if (ninst <= 0 || total_groups <= 0) return;
+    const dim3 grid((unsigned)((total_groups + 7) / 8 * 8));
+    if (adjoint) {
+        if (order == 8)
+            hipLaunchKernelGGL(k_tm_batch_adj<8>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep);
+        else if (order == 4)
+            hipLaunchKernelGGL(k_tm_batch_adj<4>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep);
+        else
+            hipLaunchKernelGGL(k_tm_batch_adj<2>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep);
+    } else {
+        if (order == 8)
+            hipLaunchKernelGGL(k_tm_batch_fwd<8>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep);
+        else if (order == 4)
+            hipLaunchKernelGGL(k_tm_batch_fwd<4>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep);
+        else
+            hipLaunchKernelGGL(k_tm_batch_fwd<2>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep);
+    }
+}
+
+void launch_tm_final_frame(const Tm2dGrid& g, const float* ez, const HistWrite& hw, hipStream_t st)
+{
+    const long long n = (long long)g.nshot * hw.box.cells;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_tm_final_frame, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, ez, hw);
+}
+
+void launch_scale(const float* in, float* out, float factor, long long count, hipStream_t st)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, in, out, factor, count);
 }
 
 // --- launchers -----------------------------------------------------------------

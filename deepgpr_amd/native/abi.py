@@ -129,6 +129,12 @@ NATIVE_FUNCTIONS: Dict[str, Tuple[tuple, object]] = {
     "dgx_set_kernel_timing": ((("enable", INT),), None),
     "dgx_kernel_timing": ((("kernel", INT), ("total_ms", VOID_PTR), ("launches", VOID_PTR)), INT),
     "dgx_kernel_name": ((("kernel", INT),), CHAR_PTR),
+    "dgx_batch_create": ((("stream", VOID_PTR),), VOID_PTR),
+    "dgx_batch_add": ((("batch", VOID_PTR), ("forward", VOID_PTR), ("gradient", VOID_PTR)), INT),
+    "dgx_batch_step": ((("batch", VOID_PTR), ("steps", INT)), INT),
+    "dgx_batch_poll": ((("batch", VOID_PTR), ("id", INT)), INT),
+    "dgx_batch_active": ((("batch", VOID_PTR),), INT),
+    "dgx_batch_destroy": ((("batch", VOID_PTR),), None),
     "deepgpr_supports_error_reporting": ((), INT),
     "deepgpr_last_error": ((), CHAR_PTR),
     "deepgpr_clear_last_error": ((), None),

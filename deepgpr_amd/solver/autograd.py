@@ -10,7 +10,7 @@ never modified.
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Any, Dict, List, Optional, Tuple
+from typing import Any, Dict, List, Optional, Sequence, Tuple
 
 import torch
 
@@ -167,6 +167,40 @@ def _own(gradient: Optional[torch.Tensor], shape, device, dtype) -> torch.Tensor
     return gradient.clone(memory_format=torch.contiguous_format)
 
 
+def forward_native_args(config: SolverConfig, tensors: SolverTensors, source_amplitudes: torch.Tensor,
+                        states: Sequence[torch.Tensor], needs_grad: bool):
+    """The dgx_forward arguments of one solve, and the arrays it fills
+    (histories, receivers). Shared by the autograd function and the batched
+    solver."""
+    dev = config.device
+    src_loc = tensors.source_location.to(device=dev, dtype=torch.int32).contiguous()
+    rx_loc = tensors.receiver_location.to(device=dev, dtype=torch.int32).contiguous()
+    if (config.save_wavefield_history and needs_grad and config.history_region == "physical"
+            and bool(config.pmlthick.any())):
+        raise ValueError("wavefield_history_region='physical' omits PML material sensitivities. "
+                         "Use 'extended' for material gradients with PML.")
+    reconstruct = bool(config.reconstruct_rhs and config.save_wavefield_history and needs_grad)
+    storage_type = encode_storage_type(config.storage, config.save_wavefield_history, rhs_from_e=reconstruct,
+                                       physical_history=config.history_region == "physical",
+                                       tm2d=config.tm2d_fast_path)
+    e_saved, r_saved, history_shape = _allocate_histories(config, needs_grad, reconstruct)
+    receivers = torch.empty((config.nstep, config.nt, config.nrx), device=dev, dtype=config.dtype)
+    args = _common(config)
+    args.update(
+        eps_r_pad=tensors.eps_r_pad, sigma_pad=tensors.sigma_pad, mu_r_pad=tensors.mu_r_pad,
+        E_saved=e_saved, R_saved=r_saved, E=list(states[0:3]), H=list(states[3:6]),
+        update_coeff=_update_coefficients(config), phi=list(states[6:30]),
+        pml_coeff=_native_pml_coefficients(tensors.pml_coefficients),
+        receiver_location=rx_loc, receiver_data=receivers, source_location=src_loc,
+        source_waveform=source_amplitudes, storage_type=storage_type,
+        save_model_history=int(config.save_wavefield_history and needs_grad),
+        use_async_offload=int(config.use_async_offload),
+    )
+    out = {"E_saved": e_saved, "R_saved": r_saved, "history_shape": history_shape, "receivers": receivers,
+           "reconstruct": reconstruct}
+    return args, out
+
+
 class DeepGPR(torch.autograd.Function):
     """``DeepGPR.apply(config, tensors, eps_r, sigma, source_amplitudes, *states)``
     returns the 30 advanced states, ``E_saved`` and the receiver data
@@ -182,18 +216,13 @@ class DeepGPR(torch.autograd.Function):
         lib = require_library(dev)
         _check_capabilities(lib, config)
         source_amplitudes = source_amplitudes.contiguous()
-        src_loc = tensors.source_location.to(device=dev, dtype=torch.int32).contiguous()
-        rx_loc = tensors.receiver_location.to(device=dev, dtype=torch.int32).contiguous()
         want_eps, want_sig = bool(eps_r.requires_grad), bool(sigma.requires_grad)
         needs_grad = want_eps or want_sig
-        if (config.save_wavefield_history and needs_grad and config.history_region == "physical"
-                and bool(config.pmlthick.any())):
-            raise ValueError("wavefield_history_region='physical' omits PML material sensitivities. "
-                             "Use 'extended' for material gradients with PML.")
-        reconstruct = bool(config.reconstruct_rhs and config.save_wavefield_history and needs_grad)
-        storage_type = encode_storage_type(config.storage, config.save_wavefield_history, rhs_from_e=reconstruct,
-                                           physical_history=config.history_region == "physical",
-                                           tm2d=config.tm2d_fast_path)
+        args, out = forward_native_args(config, tensors, source_amplitudes, states, needs_grad)
+        src_loc, rx_loc = args["source_location"], args["receiver_location"]
+        storage_type, reconstruct = args["storage_type"], out["reconstruct"]
+        e_saved, r_saved, history_shape, receivers = out["E_saved"], out["R_saved"], out["history_shape"], \
+            out["receivers"]
         if config.save_wavefield_history:
             ctx.save_for_backward(tensors.mu_r_pad, src_loc, rx_loc, *tensors.pml_coefficients,
                                   tensors.eps_r_pad, tensors.sigma_pad)
@@ -208,19 +237,6 @@ class DeepGPR(torch.autograd.Function):
         ctx.state_shapes = tuple(s.shape for s in states)
         ctx.receiver_shape = (config.nstep, config.nt, config.nrx)
 
-        e_saved, r_saved, history_shape = _allocate_histories(config, needs_grad, reconstruct)
-        receivers = torch.empty(ctx.receiver_shape, device=dev, dtype=config.dtype)
-        args = _common(config)
-        args.update(
-            eps_r_pad=tensors.eps_r_pad, sigma_pad=tensors.sigma_pad, mu_r_pad=tensors.mu_r_pad,
-            E_saved=e_saved, R_saved=r_saved, E=list(states[0:3]), H=list(states[3:6]),
-            update_coeff=_update_coefficients(config), phi=list(states[6:30]),
-            pml_coeff=_native_pml_coefficients(tensors.pml_coefficients),
-            receiver_location=rx_loc, receiver_data=receivers, source_location=src_loc,
-            source_waveform=source_amplitudes, storage_type=storage_type,
-            save_model_history=int(config.save_wavefield_history and needs_grad),
-            use_async_offload=int(config.use_async_offload),
-        )
         invoke("forward", args, dev)
         if config.debug:
             int8 = config.storage.compression == "int8"

@@ -6,6 +6,7 @@ from __future__ import annotations
 import math
 import os
 import warnings
+from dataclasses import dataclass
 from datetime import datetime
 from typing import Any, Optional, Tuple
 
@@ -131,7 +132,27 @@ def save_forward_wavefield(e_saved: torch.Tensor, directory: str, when: datetime
     return path
 
 
-def compute(device, dx=None, dt=None, source_amplitudes=None, source_location=None, receiver_location=None,
+@dataclass
+class PreparedSolve:
+    """Inputs of one solve as ``compute`` hands them to the solver."""
+
+    config: SolverConfig
+    tensors: SolverTensors
+    eps_ext: torch.Tensor
+    sig_ext: torch.Tensor
+    waveforms: torch.Tensor
+    electric: tuple
+    magnetic: tuple
+    phi: tuple
+    out_dir: Optional[str]
+    when: Optional[datetime]
+    mode: int
+    region: str
+    block: Optional[tuple]
+    sampling_interval: int
+
+
+def prepare_compute(device, dx=None, dt=None, source_amplitudes=None, source_location=None, receiver_location=None,
             er=None, se=None, mr=None, E=None, H=None, PML=None, pmlthick=DEFAULT_PML_THICKNESS,
             source_direction=DEFAULT_SOURCE_COMPONENT, reciever_direction=DEFAULT_RECEIVER_COMPONENT,
             model_gradient_sampling_interval=DEFAULT_GRADIENT_SAMPLING_INTERVAL,
@@ -144,17 +165,9 @@ def compute(device, dx=None, dt=None, source_amplitudes=None, source_location=No
             wavefield_rhs_history=DEFAULT_WAVEFIELD_RHS_HISTORY,
             wavefield_history_region=DEFAULT_WAVEFIELD_HISTORY_REGION, pml_max_velocity=SPEED_OF_LIGHT,
             gradient_accumulator=DEFAULT_GRADIENT_ACCUMULATOR):
-    """2D/3D FDTD forward modelling of Maxwell's equations, differentiable with
-    respect to ``eps_r``, ``sigma``, ``source_amplitudes`` and the initial
-    states through the exact discrete adjoint.
-
-    Every argument has the meaning and default of the reference ``compute``.
-    ``gradient_accumulator`` (keyword-only, this build) selects the precision
-    of the time-summed material gradients: ``"float64"`` (default) or
-    ``"float32"`` (bitwise the reference's accumulation).
-
-    Returns ``(E_saved, (Ex, Ey, Ez), (Hx, Hy, Hz), PML, receiver_amplitudes)``.
-    """
+    """Everything ``compute`` does before the solve: validation, model
+    extension, CPML coefficients and auxiliaries, states and the solver
+    configuration. Returns a :class:`PreparedSolve`."""
     eps_r = _alias("eps_r", eps_r, "er", er)
     sigma = _alias("sigma", sigma, "se", se)
     mu_r = _alias("mu_r", mu_r, "mr", mr)
@@ -243,14 +256,58 @@ def compute(device, dx=None, dt=None, source_amplitudes=None, source_location=No
     tensors = SolverTensors(mu_r_pad=prepared.mr, eps_r_pad=prepared.ere, sigma_pad=prepared.see,
                             source_location=src_loc, receiver_location=rx_loc, pml_descriptors=tuple(descriptors),
                             pml_coefficients=tuple(coefficients))
-    outputs = DeepGPR.apply(config, tensors, eps_ext, sig_ext, waveforms, *electric, *magnetic, *phi)
+    return PreparedSolve(config, tensors, eps_ext, sig_ext, waveforms, tuple(electric), tuple(magnetic), tuple(phi),
+                         out_dir, when, mode, region, block, S)
+
+
+def compute(device, dx=None, dt=None, source_amplitudes=None, source_location=None, receiver_location=None,
+            er=None, se=None, mr=None, E=None, H=None, PML=None, pmlthick=DEFAULT_PML_THICKNESS,
+            source_direction=DEFAULT_SOURCE_COMPONENT, reciever_direction=DEFAULT_RECEIVER_COMPONENT,
+            model_gradient_sampling_interval=DEFAULT_GRADIENT_SAMPLING_INTERVAL,
+            wavefield_storage_dtype=torch.float32, use_async_offload=False, fdtd_order=DEFAULT_FDTD_ORDER,
+            mode=DEFAULT_GRADIENT_MODE, debug=False, print_parameters=False, save_forward_wavefield_path=None, *,
+            eps_r=None, sigma=None, mu_r=None, receiver_component=None, save_wavefield_history=True,
+            wavefield_compression=DEFAULT_WAVEFIELD_COMPRESSION, wavefield_compression_block_size=None,
+            wavefield_conversion_backend=DEFAULT_CONVERSION_BACKEND,
+            int8_reduction_backend=DEFAULT_INT8_REDUCTION_BACKEND, wavefield_compression_rate=None,
+            wavefield_rhs_history=DEFAULT_WAVEFIELD_RHS_HISTORY,
+            wavefield_history_region=DEFAULT_WAVEFIELD_HISTORY_REGION, pml_max_velocity=SPEED_OF_LIGHT,
+            gradient_accumulator=DEFAULT_GRADIENT_ACCUMULATOR):
+    """2D/3D FDTD forward modelling of Maxwell's equations, differentiable with
+    respect to ``eps_r``, ``sigma``, ``source_amplitudes`` and the initial
+    states through the exact discrete adjoint.
+
+    Every argument has the meaning and default of the reference ``compute``.
+    ``gradient_accumulator`` (keyword-only, this build) selects the precision
+    of the time-summed material gradients: ``"float64"`` (default) or
+    ``"float32"`` (bitwise the reference's accumulation).
+
+    Returns ``(E_saved, (Ex, Ey, Ez), (Hx, Hy, Hz), PML, receiver_amplitudes)``.
+    """
+    p = prepare_compute(device, dx, dt, source_amplitudes, source_location, receiver_location, er, se, mr, E, H,
+                        PML, pmlthick, source_direction, reciever_direction, model_gradient_sampling_interval,
+                        wavefield_storage_dtype, use_async_offload, fdtd_order, mode, debug, print_parameters,
+                        save_forward_wavefield_path, eps_r=eps_r, sigma=sigma, mu_r=mu_r,
+                        receiver_component=receiver_component, save_wavefield_history=save_wavefield_history,
+                        wavefield_compression=wavefield_compression,
+                        wavefield_compression_block_size=wavefield_compression_block_size,
+                        wavefield_conversion_backend=wavefield_conversion_backend,
+                        int8_reduction_backend=int8_reduction_backend,
+                        wavefield_compression_rate=wavefield_compression_rate,
+                        wavefield_rhs_history=wavefield_rhs_history,
+                        wavefield_history_region=wavefield_history_region, pml_max_velocity=pml_max_velocity,
+                        gradient_accumulator=gradient_accumulator)
+    config, tensors, out_dir, when, mode, region, block, S = (p.config, p.tensors, p.out_dir, p.when, p.mode,
+                                                               p.region, p.block, p.sampling_interval)
+    nx, ny, nz, nt, nstep = config.nx, config.ny, config.nz, config.nt, config.nstep
+    outputs = DeepGPR.apply(config, tensors, p.eps_ext, p.sig_ext, p.waveforms, *p.electric, *p.magnetic, *p.phi)
     states, e_saved, receivers = outputs[:30], outputs[30], outputs[31]
 
     if out_dir is not None:
         meta = None
         origin, cells = history_box(nx, ny, nz, config.pml, region, block)
         shape = saved_history_shape(mode, saved_time_steps(nt, S), nstep, *cells)
-        if save_wavefield_history and compression == "int8":
+        if save_wavefield_history and config.storage.compression == "int8":
             meta = {"compression": "int8", "block_size": block, "uncompressed_shape": shape}
         if region == "physical":
             meta = dict(meta or {})

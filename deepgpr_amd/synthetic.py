@@ -202,3 +202,39 @@ def serving_stream(n: int, rate: float = 50.0, seed: int = 5051, nt_scale: float
         out.append(SolveRequest(eps, sig, wave, src, rx, 0.02, dt, fdtd_order=order, pmlthick=10,
                                 gradient=gradient, options={"mode": 2}, rid=rid, arrival=t))
     return out
+
+
+def batch_instances(n: int = 64, seed: int = 4041, nt_scale: float = 1.0):
+    """Independent 2D forward+adjoint solves of uneven size: nx, ny i.i.d. from
+    {128, 192, 256, 384}, order 2 or 4, nt uniform in [1000, 3000] (times
+    ``nt_scale``), CPML 10, dx 0.02 m, dt 3e-11 s; 2-6 layers along x with
+    eps_r in [1, 20] and sigma in [0, 0.01] S/m; 1-4 sources and 32-128
+    receivers on row 5; Ricker 100-500 MHz."""
+    from .serving import SolveRequest
+
+    gen = torch.Generator().manual_seed(seed)
+    sizes = (128, 192, 256, 384)
+    out = []
+    for rid in range(n):
+        nx = sizes[int(torch.randint(0, 4, (1,), generator=gen))]
+        ny = sizes[int(torch.randint(0, 4, (1,), generator=gen))]
+        order = (2, 4)[int(torch.randint(0, 2, (1,), generator=gen))]
+        nt = max(16, int(int(torch.randint(1000, 3001, (1,), generator=gen)) * nt_scale))
+        nlay = int(torch.randint(2, 7, (1,), generator=gen))
+        bounds = sorted(torch.randint(1, nx, (nlay - 1,), generator=gen).tolist())
+        eps = torch.empty((nx, ny), dtype=torch.float64)
+        sig = torch.empty((nx, ny), dtype=torch.float64)
+        for a, b in zip([0] + bounds, bounds + [nx]):
+            eps[a:b] = 1.0 + 19.0 * float(torch.rand(1, generator=gen, dtype=torch.float64))
+            sig[a:b] = 0.01 * float(torch.rand(1, generator=gen, dtype=torch.float64))
+        nsrc = int(torch.randint(1, 5, (1,), generator=gen))
+        nrx = int(torch.randint(32, 129, (1,), generator=gen))
+        sy = torch.round(torch.linspace(ny / (nsrc + 1), ny * nsrc / (nsrc + 1), nsrc)).long()
+        src = torch.stack([torch.full_like(sy, 5), sy, torch.zeros_like(sy)], dim=1).unsqueeze(0)
+        ry = torch.round(torch.linspace(0, ny - 1, nrx)).long()
+        rx = torch.stack([torch.full_like(ry, 5), ry, torch.zeros_like(ry)], dim=1).unsqueeze(0)
+        freq = 1e8 + 4e8 * float(torch.rand(1, generator=gen, dtype=torch.float64))
+        wave = ricker(freq, nt, 3e-11, 1.0 / freq, dtype=torch.float64).float().reshape(1, nt, 1)
+        out.append(SolveRequest(eps.float(), sig.float(), wave, src, rx, 0.02, 3e-11, fdtd_order=order,
+                                pmlthick=10, gradient=True, options={"mode": 2}, rid=rid))
+    return out

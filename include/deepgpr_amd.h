@@ -125,6 +125,31 @@ DGX_API int dgx_kernel_timing(int kernel, double* total_ms, long long* launches)
 /* Name of a kernel id ("" past the last id). */
 DGX_API const char* dgx_kernel_name(int kernel);
 
+/* Batched 2D Ez-TM solves. No counterpart in the reference, which batches
+ * only the shots of one model on one grid (deepgpr.cu:618, 3493). Instances of
+ * different grids, FDTD orders and numbers of time steps advance together:
+ * every dgx_batch_step issues one launch per (order, phase) for all of them.
+ * An instance is a forward solve (the dgx_solve of dgx_forward, with the
+ * SOLVER_TM2D bit, Ez source and receivers) plus, when `gradient` is not
+ * NULL, the eps_r/sigma gradient of sum(receivers^2): `gradient` supplies
+ * grad_eps_r, grad_sigma and their *_requires_grad flags, and the forward
+ * must then keep an E-only history at sampling interval 1. The instance's
+ * state arrays (E, H, phi) are consumed as scratch. All work runs on the
+ * batch's stream; one host thread drives a batch. */
+DGX_API void* dgx_batch_create(dgx_stream_t stream);
+/* Adds an instance; returns its id (> 0) or -1 (dgx_last_error). */
+DGX_API int dgx_batch_add(void* batch, const dgx_solve* forward, const dgx_solve* gradient);
+/* Advances every instance by up to `steps` time steps; returns the number of
+ * instances still running, or -1. */
+DGX_API int dgx_batch_step(void* batch, int steps);
+/* 1 when instance `id` has finished and its outputs are complete on the
+ * device (the id is then released), 0 while it runs, -1 on error. */
+DGX_API int dgx_batch_poll(void* batch, int id);
+/* Number of instances still running. */
+DGX_API int dgx_batch_active(void* batch);
+/* Waits for the batch's stream and releases everything. */
+DGX_API void dgx_batch_destroy(void* batch);
+
 /* ---------------------------------------------------------------------- */
 /* Reference ABI-6 export set (src/DeepGPR/lib/deepgpr.h:48-152)          */
 /* ---------------------------------------------------------------------- */

@@ -1,0 +1,61 @@
+"""Batched 2D TM solves: instances of different grids, orders and lengths in
+one launch per step give exactly the receivers and gradients of compute()."""
+import pytest
+import torch
+
+from deepgpr_amd import batch, serving, synthetic
+
+
+def test_batch_instances_follow_the_workload_definition():
+    reqs = synthetic.batch_instances(64, seed=4041, nt_scale=1.0)
+    assert len(reqs) == 64
+    for r in reqs:
+        assert r.eps_r.shape[0] in (128, 192, 256, 384) and r.eps_r.shape[1] in (128, 192, 256, 384)
+        assert r.fdtd_order in (2, 4) and 1000 <= r.nt <= 3000 and r.gradient
+        assert 1 <= r.source_location.shape[1] <= 4 and 32 <= r.receiver_location.shape[1] <= 128
+        assert 1.0 <= float(r.eps_r.min()) and float(r.eps_r.max()) <= 20.0
+        assert batch.batchable(r)
+    assert len({(r.eps_r.shape, r.fdtd_order, r.nt) for r in reqs}) > 50  # uneven
+
+
+def test_batchable_rejects_3d_and_int8():
+    r2 = synthetic.batch_instances(1, nt_scale=0.01)[0]
+    assert batch.batchable(r2)
+    r3 = [q for q in synthetic.serving_stream(200, nt_scale=0.01) if q.eps_r.ndim == 3][0]
+    assert not batch.batchable(r3)
+    r2.options = {"mode": 2, "wavefield_compression": "int8"}
+    assert not batch.batchable(r2)
+
+
+@pytest.mark.gpu
+def test_batch_matches_compute(cuda_device):
+    reqs = synthetic.batch_instances(6, seed=11, nt_scale=0.08)
+    reqs += [q for q in synthetic.serving_stream(30, seed=3, nt_scale=0.05) if batch.batchable(q)][:4]
+    for k, r in enumerate(reqs):
+        r.rid = k
+    reqs[-1].fdtd_order = 8
+    alone = serving.run_sequential(reqs, cuda_device)["results"]
+    with batch.BatchSolver(cuda_device) as solver:
+        ids = {solver.add(r): r.rid for r in reqs[:-2]}
+        solver.step(7)  # the last two join a running batch
+        ids.update({solver.add(r): r.rid for r in reqs[-2:]})
+        solver.run(16)
+        got = {ids[n]: solver.result(n) for n in ids}
+    for r in reqs:
+        rx, g_eps, g_sig = got[r.rid]
+        a = alone[r.rid]
+        assert torch.equal(rx, a.receivers), r.rid
+        if r.gradient:
+            assert torch.equal(g_eps, a.grad_eps_r), r.rid
+            assert torch.equal(g_sig, a.grad_sigma), r.rid
+
+
+def test_batchable_follows_compute_defaults():
+    r = synthetic.batch_instances(1, nt_scale=0.01)[0]
+    for opts, ok in [({"model_gradient_sampling_interval": 4}, False),
+                     ({"wavefield_rhs_history": "stored"}, False),
+                     ({"wavefield_storage_dtype": torch.bfloat16}, False),  # auto keeps R for lossy storage
+                     ({"wavefield_storage_dtype": torch.bfloat16, "wavefield_rhs_history": "reconstructed"}, True),
+                     ({"wavefield_rhs_history": "auto"}, True)]:
+        r.options = {"mode": 2, **opts}
+        assert batch.batchable(r) is ok, opts
