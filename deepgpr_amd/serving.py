@@ -1,9 +1,12 @@
 """Request loop for forward and forward+gradient solves of mixed-size models
 on one GPU.
 
-Requests are admitted as slots and HBM free up (continuous batching): every
-admitted request runs on its own HIP stream, so the kernels of requests in
-flight together overlap on the device, and its state (fields, CPML
+Requests are admitted as slots and HBM free up (continuous batching). 2D
+Ez-TM requests join one running batch (``batch.BatchSolver``): every time
+step of every such request in flight is one kernel launch per FDTD order and
+phase, and a request joins or leaves between two groups of steps. Other
+requests (3D, INT8 histories, ...) each run on a HIP stream of their own, so
+their kernels overlap on the device. A request's state (fields, CPML
 auxiliaries, histories) stays resident in HBM for the life of the request.
 Admission prices each request with ``estimate_compute_memory`` (the byte
 model of ``src/DeepGPR/solver/memory.py:30-268``); a request that does not fit
@@ -130,15 +133,21 @@ def solve(req: SolveRequest, device: torch.device):
 class SolveServer:
     """Continuous-batching request loop on one device (see the module docstring).
 
-    ``slots``: requests in flight at once (one stream each). ``hbm_budget``:
-    bytes the in-flight requests may hold together (default: ``hbm_fraction``
-    of the free device memory when the server starts). ``window``: how far
-    behind a request that does not fit the dispatcher looks for one that does.
+    ``slots``: requests in flight at once outside the batch (one stream each).
+    ``batch_slots``: 2D Ez-TM requests in the running batch at once (0 turns
+    the batch off; it is off with a custom ``runner`` or on the CPU).
+    ``batch_chunk``: time steps between two points where requests join or
+    leave the batch. ``batch_solver``: factory ``f(device)`` of the object
+    that runs the batch (default ``batch.BatchSolver``). ``hbm_budget``: bytes the in-flight requests may hold
+    together (default: ``hbm_fraction`` of the free device memory when the
+    server starts). ``window``: how far behind a request that does not fit the
+    dispatcher looks for one that does.
     """
 
     def __init__(self, device="cuda:0", slots: int = 8, hbm_budget: Optional[int] = None,
                  hbm_fraction: float = 0.8, window: int = 8,
-                 runner: Optional[Callable[[SolveRequest, torch.device], tuple]] = None):
+                 runner: Optional[Callable[[SolveRequest, torch.device], tuple]] = None,
+                 batch_slots: int = 64, batch_chunk: int = 32, batch_solver: Optional[Callable] = None):
         self.device = torch.device(device)
         if hbm_budget is None:
             hbm_budget = int(hbm_fraction * torch.cuda.mem_get_info(self.device)[0])
@@ -146,13 +155,24 @@ class SolveServer:
         self.slots = int(slots)
         self.window = int(window)
         self._runner = runner or solve
+        self._batch_solver = batch_solver
+        use_batch = batch_slots > 0 and (batch_solver is not None or (runner is None and self.device.type == "cuda"))
+        self.batch_slots = int(batch_slots) if use_batch else 0
+        self.batch_chunk = int(batch_chunk)
         self._pending: collections.deque = collections.deque()
         self._cv = threading.Condition()
         self._reserved = 0
         self._running = 0
+        self._batched = 0
+        self._inbox: List = []
         self._closing = False
+        self._dispatch_over = False
         self._local = threading.local()
         self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=self.slots, thread_name_prefix="dgx-slot")
+        self._batcher = None
+        if self.batch_slots:
+            self._batcher = threading.Thread(target=self._batch_loop, name="dgx-batch", daemon=True)
+            self._batcher.start()
         self._dispatcher = threading.Thread(target=self._dispatch, name="dgx-dispatch", daemon=True)
         self._dispatcher.start()
 
@@ -176,6 +196,8 @@ class SolveServer:
             self._closing = True
             self._cv.notify_all()
         self._dispatcher.join()
+        if self._batcher is not None:
+            self._batcher.join()
         self._pool.shutdown(wait=wait)
 
     def __enter__(self):
@@ -185,32 +207,108 @@ class SolveServer:
         self.close()
 
     # ------------------------------------------------------------ internals
+    def _to_batch(self, req: SolveRequest) -> bool:
+        from .batch import batchable
+        return self.batch_slots > 0 and batchable(req)
+
     def _pick(self):
-        """Index of the next admissible pending request, or None."""
-        if self._running >= self.slots:
-            return None
-        for k, (_, _, need, _) in enumerate(self._pending):
+        """(index, batched) of the next admissible pending request, or None."""
+        for k, (req, _, need, _) in enumerate(self._pending):
             if k >= self.window:
                 break
-            if self._reserved + need <= self.budget:
-                return k
+            if self._reserved + need > self.budget:
+                continue
+            batched = self._to_batch(req)
+            if (self._batched < self.batch_slots) if batched else (self._running < self.slots):
+                return k, batched
         return None
 
     def _dispatch(self) -> None:
         while True:
             with self._cv:
                 while True:
-                    k = self._pick()
-                    if k is not None or (self._closing and not self._pending):
+                    pick = self._pick()
+                    if pick is not None or (self._closing and not self._pending):
                         break
                     self._cv.wait()
-                if k is None:
+                if pick is None:
+                    self._dispatch_over = True
+                    self._cv.notify_all()
                     return
+                k, batched = pick
                 item = self._pending[k]
                 del self._pending[k]
                 self._reserved += item[2]
+                if batched:
+                    self._batched += 1
+                    self._inbox.append(item)
+                    self._cv.notify_all()
+                    continue
                 self._running += 1
             self._pool.submit(self._run, item)
+
+    def _release(self, need: int, batched: bool) -> None:
+        with self._cv:
+            self._reserved -= need
+            if batched:
+                self._batched -= 1
+            else:
+                self._running -= 1
+            self._cv.notify_all()
+
+    def _batch_loop(self) -> None:
+        """Owns the running batch: adds what the dispatcher hands over, steps
+        every instance ``batch_chunk`` time steps at a time and delivers the
+        results of finished instances."""
+        from .batch import BatchSolver
+
+        live: Dict[int, tuple] = {}
+        on_gpu = self.device.type == "cuda"
+        with torch.cuda.device(self.device) if on_gpu else contextlib.nullcontext():
+            solver = None
+            try:
+                solver = (self._batch_solver or BatchSolver)(self.device)
+                while True:
+                    with self._cv:
+                        while not self._inbox and not live and not self._dispatch_over:
+                            self._cv.wait()
+                        if not self._inbox and not live:
+                            return
+                        new, self._inbox = self._inbox, []
+                    for item in new:
+                        req, fut, need, submitted = item
+                        try:
+                            live[solver.add(req)] = (item, time.perf_counter())
+                        except BaseException as exc:  # noqa: BLE001 - delivered to the caller
+                            fut.set_exception(exc)
+                            self._release(need, True)
+                    ready = solver.step(self.batch_chunk if solver.active else 0)
+                    if not ready and live and not solver.active:
+                        solver.synchronize()  # only completion events are left
+                    for nid in ready:
+                        (req, fut, need, submitted), started = live.pop(nid)
+                        try:
+                            rx, g_eps, g_sig = solver.result(nid)
+                            fut.set_result(SolveResult(req.rid, rx, g_eps, g_sig, submitted, started,
+                                                       time.perf_counter(), need))
+                        except BaseException as exc:  # noqa: BLE001 - delivered to the caller
+                            fut.set_exception(exc)
+                        self._release(need, True)
+            except BaseException as exc:  # noqa: BLE001 - every waiting caller hears of it
+                for (req, fut, need, _), _ in live.values():
+                    if not fut.done():
+                        fut.set_exception(exc)
+                    self._release(need, True)
+                with self._cv:
+                    stranded, self._inbox = self._inbox, []
+                    self.batch_slots = 0  # later requests take the stream path
+                    self._cv.notify_all()
+                for req, fut, need, _ in stranded:
+                    fut.set_exception(exc)
+                    self._release(need, True)
+            finally:
+                if solver is not None:
+                    solver.close()
 
     def _stream(self) -> torch.cuda.Stream:
         s = getattr(self._local, "stream", None)
@@ -241,10 +339,7 @@ class SolveServer:
         except BaseException as exc:  # noqa: BLE001 - delivered to the caller
             fut.set_exception(exc)
         finally:
-            with self._cv:
-                self._reserved -= need
-                self._running -= 1
-                self._cv.notify_all()
+            self._release(need, False)
 
 
 def latency_stats(results: Sequence[SolveResult], wall: float) -> Dict:
@@ -262,11 +357,12 @@ def latency_stats(results: Sequence[SolveResult], wall: float) -> Dict:
 
 
 def run_stream(requests: Sequence[SolveRequest], device="cuda:0", slots: int = 8, paced: bool = True,
-               hbm_budget: Optional[int] = None) -> Dict:
+               hbm_budget: Optional[int] = None, batch_slots: int = 64, batch_chunk: int = 32) -> Dict:
     """Feeds requests to a server at their arrival times (or all at once when
     ``paced`` is False); returns the stats and the results by request id."""
     results: Dict[int, SolveResult] = {}
-    with SolveServer(device, slots=slots, hbm_budget=hbm_budget) as server:
+    with SolveServer(device, slots=slots, hbm_budget=hbm_budget, batch_slots=batch_slots,
+                     batch_chunk=batch_chunk) as server:
         t0 = time.perf_counter()
         futs = []
         for req in requests:
@@ -318,6 +414,9 @@ def _main() -> int:
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--seed", type=int, default=5051)
     ap.add_argument("--scale", type=float, default=1.0, help="multiplies every nt (shorter runs)")
+    ap.add_argument("--batch-slots", type=int, default=64, help="2D requests in the running batch (0: off)")
+    ap.add_argument("--batch-chunk", type=int, default=32, help="time steps between joins")
+    ap.add_argument("--streams-only", action="store_true", help="also run the server with the batch off")
     args = ap.parse_args()
     reqs = synthetic.serving_stream(args.requests, rate=args.rate, seed=args.seed, nt_scale=args.scale)
     dev = "cuda:0"
@@ -325,17 +424,22 @@ def _main() -> int:
     warm = synthetic.serving_stream(1, rate=args.rate, seed=args.seed + 1, nt_scale=0.05)
     run_sequential(warm, dev)
     seq = run_sequential(reqs, dev)
-    paced = run_stream(reqs, dev, slots=args.slots, paced=True)
-    burst = run_stream(reqs, dev, slots=args.slots, paced=False)
+    kw = dict(slots=args.slots, batch_slots=args.batch_slots, batch_chunk=args.batch_chunk)
+    paced = run_stream(reqs, dev, paced=True, **kw)
+    burst = run_stream(reqs, dev, paced=False, **kw)
     worst = 0.0
     for rid, r in burst["results"].items():
         ref = seq["results"][rid]
         worst = max(worst, float((r.receivers - ref.receivers).abs().max()))
     line = {"metric": "serve p50/p99 ms vs plain solve", "requests": len(reqs), "rate_per_s": args.rate,
-            "slots": args.slots, "nt_scale": args.scale, "sequential": seq["stats"], "served_paced": paced["stats"],
+            "slots": args.slots, "batch_slots": args.batch_slots, "batch_chunk": args.batch_chunk,
+            "nt_scale": args.scale, "sequential": seq["stats"], "served_paced": paced["stats"],
             "served_saturated": burst["stats"],
             "throughput_ratio": round(burst["stats"]["gcell_updates_per_s"] / seq["stats"]["gcell_updates_per_s"], 3),
             "max_abs_receiver_diff_vs_alone": worst}
+    if args.streams_only:
+        alt = run_stream(reqs, dev, paced=False, slots=args.slots, batch_slots=0)
+        line["served_saturated_streams_only"] = alt["stats"]
     print(json.dumps(line), flush=True)
     return 0
 

@@ -1,5 +1,6 @@
 """Admission, slot and ordering behaviour of the request loop (CPU, fake solver),
 and GPU checks that a result is the same alone and in the mix."""
+import concurrent.futures
 import threading
 import time
 
@@ -43,6 +44,88 @@ class _Fake:
             self.running -= 1
             self.held -= need
         return torch.full((1, req.nt, 2), float(req.rid)), None, None
+
+
+class _FakeBatch:
+    """Batch runner: an instance finishes once ``step`` has advanced it nt steps."""
+
+    made = []
+
+    def __init__(self, device, fail_rid=None):
+        self.left, self.reqs, self.next, self.peak, self.closed = {}, {}, 0, 0, False
+        self.fail_rid = fail_rid
+        self.seen = []
+        _FakeBatch.made.append(self)
+
+    def add(self, req):
+        if req.rid == self.fail_rid:
+            raise ValueError("cannot batch this one")
+        nid, self.next = self.next, self.next + 1
+        self.left[nid], self.reqs[nid] = req.nt, req
+        self.peak = max(self.peak, len(self.left))
+        self.seen.append(req.rid)
+        return nid
+
+    def step(self, steps):
+        time.sleep(0.002 if steps else 0.0)  # a group of steps takes device time
+        ready = [n for n in self.left if self.left[n] <= steps]
+        for n in list(self.left):
+            self.left[n] -= steps
+            if self.left[n] <= 0:
+                del self.left[n]
+        return ready
+
+    @property
+    def active(self):
+        return len(self.left)
+
+    def synchronize(self):
+        pass
+
+    def result(self, nid):
+        req = self.reqs.pop(nid)
+        return torch.full((1, req.nt, 2), float(req.rid)), None, None
+
+    def close(self):
+        self.closed = True
+
+
+def _req3d(rid, side=32, nt=40):
+    eps = torch.full((side, side, side), 4.0)
+    wave = torch.zeros((1, nt, 1))
+    src = torch.tensor([[[5, side // 2, side // 2]]])
+    rx = torch.tensor([[[5, 3, 3]]])
+    return serving.SolveRequest(eps, torch.zeros_like(eps), wave, src, rx, 0.02, 2.5e-11, rid=rid)
+
+
+@pytest.mark.parametrize("fail_rid", [None, 4])
+def test_batchable_requests_share_the_batch_and_the_rest_take_streams(fail_rid):
+    _FakeBatch.made.clear()
+    fake = _Fake(hold=0.01)
+    reqs = []
+    for k in range(20):
+        r = _req3d(k) if k % 5 == 0 else _req(k, 128, nt=30 + 17 * k)
+        if k == 7:
+            r.options = {"model_gradient_sampling_interval": 3}  # not batchable: stream path
+        reqs.append(r)
+    with serving.SolveServer("cpu", slots=2, hbm_budget=1 << 40, runner=fake, batch_slots=6, batch_chunk=8,
+                             batch_solver=lambda dev: _FakeBatch(dev, fail_rid)) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        done = concurrent.futures.wait(futs, timeout=30)
+        assert not done.not_done
+    (batch,) = _FakeBatch.made
+    assert batch.closed and 1 < batch.peak <= 6
+    streamed = sorted(fake.order)
+    assert streamed == [0, 5, 7, 10, 15]
+    assert sorted(batch.seen + streamed + ([fail_rid] if fail_rid else [])) == list(range(20))
+    for r, f in zip(reqs, futs):
+        if r.rid == fail_rid:
+            with pytest.raises(ValueError):
+                f.result()
+            continue
+        res = f.result()
+        assert res.rid == r.rid and float(res.receivers[0, 0, 0]) == r.rid
+        assert res.receivers.shape[1] == r.nt
 
 
 def test_footprint_grows_with_grid_and_gradient():
@@ -97,12 +180,18 @@ def test_serving_stream_mix():
 
 
 @pytest.mark.gpu
-def test_results_same_alone_and_in_the_mix(cuda_device):
+@pytest.mark.parametrize("batch_slots", [0, 3])
+def test_results_same_alone_and_in_the_mix(cuda_device, batch_slots):
     reqs = synthetic.serving_stream(10, rate=200.0, seed=7, nt_scale=0.05)
     reqs = [r for r in reqs if r.eps_r.ndim == 2][:6]
     reqs[1].gradient = True
+    reqs[2].options = {"mode": 2, "model_gradient_sampling_interval": 2}  # not batchable
+    reqs[2].gradient = True
+    reqs.append([r for r in synthetic.serving_stream(200, seed=8, nt_scale=0.02) if r.eps_r.ndim == 3][0])
+    reqs[-1].rid = 99
     alone = serving.run_sequential(reqs, cuda_device)["results"]
-    mixed = serving.run_stream(reqs, cuda_device, slots=4, paced=False)["results"]
+    mixed = serving.run_stream(reqs, cuda_device, slots=2, paced=False, batch_slots=batch_slots,
+                               batch_chunk=5)["results"]
     for r in reqs:
         a, m = alone[r.rid], mixed[r.rid]
         assert torch.equal(a.receivers, m.receivers), r.rid
