@@ -17,7 +17,8 @@ A gradient instance asks for the eps_r / sigma gradient of
 
 ``python -m deepgpr_amd.batch`` runs the 64-instance batch of
 ``synthetic.batch_instances`` one at a time through ``compute`` and as one
-batch, checks that the results agree and prints one JSON line.
+batch (instances join while the batch steps, longest first), checks that the
+results agree and prints one JSON line.
 """
 from __future__ import annotations
 
@@ -71,7 +72,10 @@ class BatchSolver:
         self.lookahead = int(lookahead)  # step() calls the host may run ahead of the device
         self._marks: collections.deque = collections.deque()
         self._lib = require_library(self.device)
-        self.stream = stream or torch.cuda.Stream(self.device)  # default priority
+        self.stream = stream or torch.cuda.Stream(self.device)  # default priority; time steps
+        # requests are prepared and results read back on a second stream, so
+        # that their host synchronisations do not wait for queued time steps
+        self.side = torch.cuda.Stream(self.device)
         self._handle = self._lib.dgx_batch_create(ctypes.c_void_p(self.stream.cuda_stream))
         if not self._handle:
             raise NativeLibraryError(self._error())
@@ -88,7 +92,7 @@ class BatchSolver:
         if not batchable(req):
             raise ValueError("batched solves are 2D Ez-TM solves (mode 2, no INT8 history)")
         dev = self.device
-        with torch.cuda.device(dev), torch.cuda.stream(self.stream):
+        with torch.cuda.device(dev), torch.cuda.stream(self.side):
             eps = req.eps_r.to(dev)
             sig = req.sigma.to(dev)
             if req.gradient:
@@ -117,7 +121,8 @@ class BatchSolver:
                 grad = build_solve({"grad_eps_r": g_eps, "grad_sigma": g_sig, "eps_r_requires_grad": 1,
                                     "sigma_requires_grad": 1})
             nid = self._lib.dgx_batch_add(self._handle, ctypes.byref(fwd),
-                                          ctypes.byref(grad) if grad is not None else None)
+                                          ctypes.byref(grad) if grad is not None else None,
+                                          ctypes.c_void_p(self.side.cuda_stream))
             if nid < 0:
                 raise NativeLibraryError(self._error())
         self._running[nid] = _Instance(req.rid, req.gradient, out["receivers"], eps, sig, p.eps_ext, p.sig_ext,
@@ -167,8 +172,8 @@ class BatchSolver:
         """(receivers, grad_eps_r, grad_sigma) of a finished instance, on the
         CPU; gradients are folded onto the physical model (None for a
         forward-only instance)."""
-        inst = self._done.pop(nid)
-        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+        inst = self._done.pop(nid)  # its completion event has passed
+        with torch.cuda.device(self.device), torch.cuda.stream(self.side):
             g_eps = g_sig = None
             if inst.gradient:
                 g_eps, g_sig = torch.autograd.grad([inst.eps_ext, inst.sig_ext], [inst.eps, inst.sig],
@@ -209,7 +214,11 @@ def _main() -> int:
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     with BatchSolver(dev) as solver:
-        ids = {solver.add(r): r.rid for r in reqs}
+        # longest first, and the device steps while the host prepares the next
+        ids = {}
+        for r in sorted(reqs, key=lambda q: (2 if q.gradient else 1) * q.nt, reverse=True):
+            ids[solver.add(r)] = r.rid
+            solver.step(args.chunk)
         solver.run(args.chunk)
         results = {ids[n]: solver.result(n) for n in ids}
     torch.cuda.synchronize(dev)

@@ -58,6 +58,10 @@ struct DevBuf {
         st = g_call_stream;
         return hipMallocAsync(&p, bytes, st) == hipSuccess;
     }
+    // The free follows the work queued on `s` instead: the stream that uses
+    // the buffer last (a free on any other stream could return the memory
+    // while kernels queued on `s` still read it).
+    void free_after(hipStream_t s) { st = s; }
 };
 
 #define DGX_TRY(call, what)                                                         \
@@ -1310,6 +1314,22 @@ struct BatchInstance {
     long long ngrad = 0;
     const float* rx_out = nullptr;
     long long rx_count = 0;
+    // set up on another stream, used and finally freed on the batch stream
+    void free_after(hipStream_t s)
+    {
+        for (DevBuf* d : {&tms.buf, &tms.alt, &l_soff, &l_sent, &l_roff, &l_rent, &l_xoff, &l_xgrp, &l_mem, &dgrad,
+                          &acc, &terms})
+            d->free_after(s);
+    }
+};
+
+// Pinned host buffer that stages one descriptor upload; `done` marks the end
+// of the copy that last read it, so the host waits only for that copy (a
+// pageable copy would wait for everything queued on the stream before it).
+struct Staging {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
 };
 
 struct Batch {
@@ -1319,17 +1339,30 @@ struct Batch {
     std::vector<std::unique_ptr<BatchInstance>> active;
     std::vector<std::pair<int, hipEvent_t>> finished;
     bool dirty = true;
-    DevBuf desc[6], prefix[6];
-    size_t desc_cap[6] = {}, prefix_cap[6] = {};
+    DevBuf table;  // descriptors and prefixes of the 6 slots
+    size_t table_cap = 0;
+    Staging ring[8];
+    int ring_next = 0;
+    const TmBatchInst* desc[6] = {};
+    const int* prefix[6] = {};
     int ninst[6] = {}, total[6] = {}, order_of[6] = {2, 4, 8, 2, 4, 8};
+    ~Batch()
+    {
+        for (Staging& r : ring) {
+            if (r.done) (void)hipEventDestroy(r.done);
+            if (r.host) (void)hipHostFree(r.host);
+        }
+    }
 };
 
 int order_slot(int order) { return order == 8 ? 2 : (order == 4 ? 1 : 0); }
 
-int batch_add(Batch* b, const dgx_solve* P, const dgx_solve* G)
+// Sets an instance up on `stream` (not the batch stream, so that the host
+// waits only for the set-up work); the batch stream waits for it before the
+// instance's first step.
+int batch_add(Batch* b, const dgx_solve* P, const dgx_solve* G, hipStream_t stream)
 {
-    CallStream call(b->stream);
-    hipStream_t stream = b->stream;
+    CallStream call(stream);
     Plan pl;
     if (make_plan(P, false, pl)) return -1;
     const Grid& g = pl.g;
@@ -1453,41 +1486,74 @@ int batch_add(Batch* b, const dgx_solve* P, const dgx_solve* G)
     }
     d.phase = 0;
     d.start = b->gstep;
+    DGX_TRY(hipGetLastError(), "batch: kernel launch");
+    if (stream != b->stream) {
+        hipEvent_t ready;
+        DGX_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "batch: event");
+        hipError_t e = hipEventRecord(ready, stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, ready, 0);
+        (void)hipEventDestroy(ready);
+        DGX_TRY(e, "batch: order the set-up before the batch");
+        I.free_after(b->stream);
+    }
     b->active.push_back(std::move(inst));
     b->dirty = true;
-    DGX_TRY(hipGetLastError(), "batch: kernel launch");
     return b->active.back()->id;
 }
 
 int batch_rebuild(Batch* b)
 {
+    // one table: per slot, 256-byte aligned descriptors then prefix sums
+    std::vector<TmBatchInst> desc[6];
+    std::vector<int> prefix[6];
+    size_t off_desc[6], off_prefix[6], bytes = 0;
+    auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
     for (int slot = 0; slot < 6; ++slot) {
-        std::vector<TmBatchInst> desc;
-        std::vector<int> prefix(1, 0);
+        prefix[slot].assign(1, 0);
         for (auto& ip : b->active) {
             const BatchInstance& I = *ip;
             if (order_slot(I.order) + 3 * I.d.phase != slot) continue;
-            desc.push_back(I.d);
-            prefix.push_back(prefix.back() + I.d.g.ftiles * I.d.g.nshot);
+            desc[slot].push_back(I.d);
+            prefix[slot].push_back(prefix[slot].back() + I.d.g.ftiles * I.d.g.nshot);
         }
-        b->ninst[slot] = (int)desc.size();
-        b->total[slot] = prefix.back();
-        if (desc.empty()) continue;
-        const size_t db = desc.size() * sizeof(TmBatchInst), pb = prefix.size() * sizeof(int);
-        if (db > b->desc_cap[slot]) {
-            b->desc[slot].reset();
-            if (!b->desc[slot].alloc(db)) return -1;
-            b->desc_cap[slot] = db;
-        }
-        if (pb > b->prefix_cap[slot]) {
-            b->prefix[slot].reset();
-            if (!b->prefix[slot].alloc(pb)) return -1;
-            b->prefix_cap[slot] = pb;
-        }
-        if (hipMemcpyAsync(b->desc[slot].p, desc.data(), db, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-            hipMemcpyAsync(b->prefix[slot].p, prefix.data(), pb, hipMemcpyHostToDevice, b->stream) != hipSuccess)
-            return -1;
+        b->ninst[slot] = (int)desc[slot].size();
+        b->total[slot] = prefix[slot].back();
+        off_desc[slot] = bytes;
+        bytes = align(bytes + desc[slot].size() * sizeof(TmBatchInst));
+        off_prefix[slot] = bytes;
+        bytes = align(bytes + prefix[slot].size() * sizeof(int));
     }
+    if (bytes > b->table_cap) {
+        b->table.reset();  // stream-ordered: earlier steps finish reading it first
+        const size_t cap = std::max(bytes, 2 * b->table_cap);
+        if (!b->table.alloc(cap)) return -1;
+        b->table_cap = cap;
+    }
+    Staging& r = b->ring[b->ring_next];
+    b->ring_next = (b->ring_next + 1) % 8;
+    if (r.done && hipEventSynchronize(r.done) != hipSuccess) return -1;
+    if (bytes > r.cap) {
+        if (r.host) (void)hipHostFree(r.host);
+        r.host = nullptr;
+        const size_t cap = std::max(bytes, (size_t)64 << 10);
+        if (hipHostMalloc(&r.host, cap, hipHostMallocDefault) != hipSuccess) {
+            r.cap = 0;
+            return -1;
+        }
+        r.cap = cap;
+    }
+    if (!r.done && hipEventCreateWithFlags(&r.done, hipEventDisableTiming) != hipSuccess) return -1;
+    char* h = (char*)r.host;
+    char* dv = (char*)b->table.p;
+    for (int slot = 0; slot < 6; ++slot) {
+        std::memcpy(h + off_desc[slot], desc[slot].data(), desc[slot].size() * sizeof(TmBatchInst));
+        std::memcpy(h + off_prefix[slot], prefix[slot].data(), prefix[slot].size() * sizeof(int));
+        b->desc[slot] = (const TmBatchInst*)(dv + off_desc[slot]);
+        b->prefix[slot] = (const int*)(dv + off_prefix[slot]);
+    }
+    if (hipMemcpyAsync(dv, h, bytes, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipEventRecord(r.done, b->stream) != hipSuccess)
+        return -1;
     b->dirty = false;
     return 0;
 }
@@ -1510,18 +1576,22 @@ int batch_phase_end(Batch* b, BatchInstance& I, bool& finished)
         hw.e_cstride = d.snap;
         launch_tm_final_frame(d.g, ez, hw, stream);
         launch_scale(I.rx_out, (float*)I.dgrad.p, 2.0f, I.rx_count, stream);
-        // the adjoint starts from zero cotangents in both ping-pong copies
-        for (int k = 0; k < 2; ++k) {
-            DGX_TRY(hipMemsetAsync(d.pp.Ez[k], 0, field * 4, stream), "batch: clear");
-            DGX_TRY(hipMemsetAsync(d.pp.Hx[k], 0, field * 4, stream), "batch: clear");
-            DGX_TRY(hipMemsetAsync(d.pp.Hy[k], 0, field * 4, stream), "batch: clear");
-            for (int f4 = 0; f4 < 4; ++f4) {
-                if (I.tms.psi_len_e[f4])
-                    DGX_TRY(hipMemsetAsync(d.pp.psi_e[k][f4], 0, I.tms.psi_len_e[f4] * 4, stream), "batch: clear");
-                if (I.tms.psi_len_h[f4])
-                    DGX_TRY(hipMemsetAsync(d.pp.psi_h[k][f4], 0, I.tms.psi_len_h[f4] * 4, stream), "batch: clear");
-            }
+        // the adjoint starts from zero cotangents in both ping-pong copies:
+        // the six field copies are one block of the state buffer, the CPML
+        // auxiliaries are the home arrays plus one block of second copies
+        ClearSpans sp{};
+        bool fits = sp.add((float*)I.tms.buf.p, 6 * field);
+        long long alt_len = 0;
+        for (int f4 = 0; f4 < 4; ++f4) {
+            fits = fits && sp.add(I.tms.home_e[f4], I.tms.psi_len_e[f4]) && sp.add(I.tms.home_h[f4], I.tms.psi_len_h[f4]);
+            alt_len += I.tms.psi_len_e[f4] + I.tms.psi_len_h[f4];
         }
+        fits = fits && sp.add((float*)I.tms.alt.p, alt_len);
+        if (!fits) {
+            set_error("batch: too many arrays to clear");
+            return -1;
+        }
+        launch_clear_spans(sp, stream);
         d.phase = 1;
         d.start = b->gstep;
         return 0;
@@ -1546,8 +1616,8 @@ int batch_step(Batch* b, int steps)
             return -1;
         }
         for (int slot = 0; slot < 6; ++slot)
-            launch_tm_batch(b->order_of[slot], slot >= 3, (const TmBatchInst*)b->desc[slot].p,
-                            (const int*)b->prefix[slot].p, b->ninst[slot], b->total[slot], b->gstep, b->stream);
+            launch_tm_batch(b->order_of[slot], slot >= 3, b->desc[slot], b->prefix[slot], b->ninst[slot],
+                            b->total[slot], b->gstep, b->stream);
         b->gstep += 1;
         for (size_t q = 0; q < b->active.size();) {
             BatchInstance& I = *b->active[q];
@@ -1597,14 +1667,16 @@ DGX_API void* dgx_batch_create(dgx_stream_t stream)
     return b;
 }
 
-DGX_API int dgx_batch_add(void* batch, const dgx_solve* forward, const dgx_solve* gradient)
+DGX_API int dgx_batch_add(void* batch, const dgx_solve* forward, const dgx_solve* gradient,
+                          dgx_stream_t setup_stream)
 {
     g_error[0] = '\0';
     if (!batch || !forward) {
         set_error("batch: null argument");
         return -1;
     }
-    return batch_add((Batch*)batch, forward, gradient);
+    Batch* b = (Batch*)batch;
+    return batch_add(b, forward, gradient, setup_stream ? (hipStream_t)setup_stream : b->stream);
 }
 
 DGX_API int dgx_batch_step(void* batch, int steps)

@@ -206,6 +206,23 @@ void launch_tm_batch(int order, bool adjoint, const TmBatchInst* insts, const in
                      int total_groups, int gstep, hipStream_t st);
 void launch_tm_final_frame(const Tm2dGrid& g, const float* ez, const HistWrite& hw, hipStream_t st);
 void launch_scale(const float* in, float* out, float factor, long long count, hipStream_t st);
+// Float arrays to zero in one launch (a fill kernel, not hipMemsetAsync).
+constexpr int kMaxSpans = 16;
+struct ClearSpans {
+    float* p[kMaxSpans];
+    long long n[kMaxSpans];
+    int count;
+    bool add(float* ptr, long long len)
+    {
+        if (len <= 0) return true;
+        if (count >= kMaxSpans) return false;
+        p[count] = ptr;
+        n[count] = len;
+        ++count;
+        return true;
+    }
+};
+void launch_clear_spans(const ClearSpans& sp, hipStream_t st);
 
 void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
                         const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,

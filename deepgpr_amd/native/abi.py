@@ -130,7 +130,8 @@ NATIVE_FUNCTIONS: Dict[str, Tuple[tuple, object]] = {
     "dgx_kernel_timing": ((("kernel", INT), ("total_ms", VOID_PTR), ("launches", VOID_PTR)), INT),
     "dgx_kernel_name": ((("kernel", INT),), CHAR_PTR),
     "dgx_batch_create": ((("stream", VOID_PTR),), VOID_PTR),
-    "dgx_batch_add": ((("batch", VOID_PTR), ("forward", VOID_PTR), ("gradient", VOID_PTR)), INT),
+    "dgx_batch_add": ((("batch", VOID_PTR), ("forward", VOID_PTR), ("gradient", VOID_PTR),
+                       ("setup_stream", VOID_PTR)), INT),
     "dgx_batch_step": ((("batch", VOID_PTR), ("steps", INT)), INT),
     "dgx_batch_poll": ((("batch", VOID_PTR), ("id", INT)), INT),
     "dgx_batch_active": ((("batch", VOID_PTR),), INT),

@@ -102,9 +102,13 @@ typedef struct dgx_solve {
 DGX_API int dgx_abi_version(void);
 /* Message of the last failed call on this thread ("" after success). */
 DGX_API const char* dgx_last_error(void);
-/* Forward solve; replaces forward() of deepgpr.h:89-115. */
+/* Forward solve; replaces forward() of deepgpr.h:89-115. Stream-ordered:
+ * the call queues its work (and the free of its scratch) on `stream` and
+ * returns without waiting, unless kernel timing is on; the caller's arrays
+ * must stay valid until that work has run. */
 DGX_API int dgx_forward(const dgx_solve* solve, dgx_stream_t stream);
-/* Exact discrete adjoint; replaces backward() of deepgpr.h:117-152. */
+/* Exact discrete adjoint; replaces backward() of deepgpr.h:117-152.
+ * Stream-ordered like dgx_forward. */
 DGX_API int dgx_backward(const dgx_solve* solve, dgx_stream_t stream);
 /* FP16/BF16 history codec round trip on device arrays; replaces
  * deepgpr_test_wavefield_conversion (deepgpr.h:84-86). */
@@ -134,11 +138,16 @@ DGX_API const char* dgx_kernel_name(int kernel);
  * NULL, the eps_r/sigma gradient of sum(receivers^2): `gradient` supplies
  * grad_eps_r, grad_sigma and their *_requires_grad flags, and the forward
  * must then keep an E-only history at sampling interval 1. The instance's
- * state arrays (E, H, phi) are consumed as scratch. All work runs on the
+ * state arrays (E, H, phi) are consumed as scratch. Time steps run on the
  * batch's stream; one host thread drives a batch. */
 DGX_API void* dgx_batch_create(dgx_stream_t stream);
-/* Adds an instance; returns its id (> 0) or -1 (dgx_last_error). */
-DGX_API int dgx_batch_add(void* batch, const dgx_solve* forward, const dgx_solve* gradient);
+/* Adds an instance; returns its id (> 0) or -1 (dgx_last_error). Its set-up
+ * (coefficients, lists, buffers) runs on `setup_stream` (NULL: the batch's
+ * stream), which must be the stream that produced the instance's inputs; the
+ * batch's stream waits for it before the instance's first step, and the host
+ * waits only for the set-up stream. */
+DGX_API int dgx_batch_add(void* batch, const dgx_solve* forward, const dgx_solve* gradient,
+                          dgx_stream_t setup_stream);
 /* Advances every instance by up to `steps` time steps; returns the number of
  * instances still running, or -1. */
 DGX_API int dgx_batch_step(void* batch, int steps);
