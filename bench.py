@@ -161,6 +161,22 @@ def kernel_bytes(kernel: str, w, args) -> float:
     return 0.0
 
 
+# HBM bytes per launch of a dominant kernel, measured with rocprofv3 in two
+# runs (--pmc FETCH_SIZE, then --pmc WRITE_SIZE) and scaled by what the same
+# counters read for the 256 MiB device copy bench.py --copy-calibration-mb runs
+# first (reads x2.0, writes x1.0). Keyed by (workload, kernel, history dtype,
+# accumulator, shots per call): traffic depends on all of them. The commands
+# and per-kernel tables are in the profiles/*.md file named with each entry.
+MEASURED_TRAFFIC = {
+    ("c2", "tm_step_adj", "float32", "float64", 20): (121.3e6, "profiles/c2_fwi_2d_o4_tm_step_kernels.md"),
+    ("c2", "tm_step_fwd", "float32", "float64", 20): (90.4e6, "profiles/c2_fwi_2d_o4_tm_step_kernels.md"),
+    ("c3", "fwd_e", "float32", "float64", 4): (6781.1e6, "profiles/c3_layered_3d_o4_kernels.md"),
+    ("c3", "fwd_h", "float32", "float64", 4): (5639.3e6, "profiles/c3_layered_3d_o4_kernels.md"),
+    ("c3", "adj_h", "float32", "float64", 4): (5893.9e6, "profiles/c3_layered_3d_o4_kernels.md"),
+    ("c3", "adj_e", "float32", "float64", 4): (5554.5e6, "profiles/c3_layered_3d_o4_kernels.md"),
+}
+
+
 def _shots_per_call(w, args, free_bytes: int) -> int:
     """Shots per compute() call: all of them when the whole solve fits in 85 %
     of free HBM, else as many as fit (a history larger than HBM runs shot by shot)."""
@@ -313,9 +329,13 @@ def main():
             avg_s = tot / cnt / 1e3
             bytes_launch = kernel_bytes(dom, w, args)
             achieved = bytes_launch / avg_s / 1e9
+            measured = MEASURED_TRAFFIC.get((args.workload, dom, args.storage, args.accumulator, chunk))
             roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": int(measured[0]) if measured else None,
                         "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_launch)}
+            if measured:
+                roofline["traffic_source"] = measured[1]
     fwd_b, adj_b = model_bytes_per_cell(w, args)
     pair_bytes = fwd_b + adj_b
     cpu = None if args.no_cpu_baseline else _cpu_baseline(args)

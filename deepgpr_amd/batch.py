@@ -5,7 +5,7 @@ in the C ABI; the reference batches only the shots of one model,
 ``deepgpr.cu:618, 3493``).
 
     solver = BatchSolver("cuda:0")
-    ids = [solver.add(req) for req in requests]     # serving.SolveRequest, 2D
+    ids = [solver.submit(req) for req in requests]  # serving.SolveRequest, 2D
     solver.run()                                    # until every instance finished
     rx, g_eps, g_sig = solver.result(ids[0])        # physical-grid gradients
 
@@ -17,13 +17,16 @@ A gradient instance asks for the eps_r / sigma gradient of
 
 ``python -m deepgpr_amd.batch`` runs the 64-instance batch of
 ``synthetic.batch_instances`` one at a time through ``compute`` and as one
-batch (instances join while the batch steps, longest first), checks that the
-results agree and prints one JSON line.
+batch (prepared on a background thread, longest first, joining while the
+batch steps), checks that the results agree and prints one JSON line.
 """
 from __future__ import annotations
 
 import collections
 import ctypes
+import itertools
+import queue
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -64,8 +67,21 @@ def batchable(req) -> bool:
             and (rhs == "reconstructed" or (rhs == "auto" and lossless)))
 
 
+@dataclass
+class _Prepared:
+    req: object
+    fwd: object
+    grad: object
+    inst: _Instance
+
+
 class BatchSolver:
-    """Drives one ``dgx_batch`` on its own stream from one host thread."""
+    """Drives one ``dgx_batch`` on its own stream from one host thread.
+
+    ``add`` prepares a request on the calling thread and puts it in the
+    batch. ``submit`` hands it to a background thread that prepares it while
+    the batch steps; it joins at a later ``step``. Both return an id that
+    ``step`` reports once the result is complete and ``result`` reads."""
 
     def __init__(self, device="cuda:0", stream: Optional[torch.cuda.Stream] = None, lookahead: int = 2):
         self.device = torch.device(device)
@@ -79,16 +95,20 @@ class BatchSolver:
         self._handle = self._lib.dgx_batch_create(ctypes.c_void_p(self.stream.cuda_stream))
         if not self._handle:
             raise NativeLibraryError(self._error())
-        self._running: Dict[int, _Instance] = {}
-        self._done: Dict[int, _Instance] = {}
+        self._ids = itertools.count(1)
+        self._running: Dict[int, tuple] = {}  # id -> (native id, _Instance)
+        self._done: Dict[int, object] = {}    # id -> _Instance or the exception that ended it
+        self._pending = 0                     # submitted, not yet in the batch
+        self._inbox: queue.Queue = queue.Queue()
+        self._prepared: collections.deque = collections.deque()
+        self._worker: Optional[threading.Thread] = None
 
     def _error(self) -> str:
         msg = self._lib.dgx_last_error()
         return msg.decode() if isinstance(msg, bytes) else str(msg)
 
-    # ------------------------------------------------------------------ API
-    def add(self, req) -> int:
-        """Prepares a request and adds it to the batch; returns its id."""
+    # ---------------------------------------------------------- preparation
+    def _prepare(self, req) -> _Prepared:
         if not batchable(req):
             raise ValueError("batched solves are 2D Ez-TM solves (mode 2, no INT8 history)")
         dev = self.device
@@ -120,36 +140,89 @@ class BatchSolver:
                 g_sig = torch.zeros(cells, device=dev, dtype=p.config.dtype)
                 grad = build_solve({"grad_eps_r": g_eps, "grad_sigma": g_sig, "eps_r_requires_grad": 1,
                                     "sigma_requires_grad": 1})
-            nid = self._lib.dgx_batch_add(self._handle, ctypes.byref(fwd),
-                                          ctypes.byref(grad) if grad is not None else None,
+        inst = _Instance(req.rid, req.gradient, out["receivers"], eps, sig, p.eps_ext, p.sig_ext, g_eps, g_sig,
+                         keep=[args, out, p, fwd, grad, amp])
+        return _Prepared(req, fwd, grad, inst)
+
+    def _commit(self, pid: int, prep: _Prepared) -> None:
+        with torch.cuda.device(self.device):
+            nid = self._lib.dgx_batch_add(self._handle, ctypes.byref(prep.fwd),
+                                          ctypes.byref(prep.grad) if prep.grad is not None else None,
                                           ctypes.c_void_p(self.side.cuda_stream))
-            if nid < 0:
-                raise NativeLibraryError(self._error())
-        self._running[nid] = _Instance(req.rid, req.gradient, out["receivers"], eps, sig, p.eps_ext, p.sig_ext,
-                                       g_eps, g_sig, keep=[args, out, p, fwd, grad, amp])
-        return nid
+        if nid < 0:
+            raise NativeLibraryError(self._error())
+        self._running[pid] = (nid, prep.inst)
+
+    def _work(self) -> None:
+        while True:
+            item = self._inbox.get()
+            if item is None:
+                return
+            pid, req = item
+            try:
+                self._prepared.append((pid, self._prepare(req)))
+            except BaseException as exc:  # noqa: BLE001 - reported through result()
+                self._prepared.append((pid, exc))
+
+    # ------------------------------------------------------------------ API
+    def add(self, req) -> int:
+        """Prepares a request on this thread and adds it to the batch; returns its id."""
+        pid = next(self._ids)
+        self._commit(pid, self._prepare(req))
+        return pid
+
+    def submit(self, req) -> int:
+        """Queues a request for preparation on the background thread; it joins
+        the batch at a later step(). Returns its id."""
+        if self._worker is None:
+            self._worker = threading.Thread(target=self._work, name="dgx-batch-prepare", daemon=True)
+            self._worker.start()
+        pid = next(self._ids)
+        self._pending += 1
+        self._inbox.put((pid, req))
+        return pid
+
+    @property
+    def pending(self) -> int:
+        """Submitted requests not yet in the batch."""
+        return self._pending
 
     def step(self, steps: int = 1) -> List[int]:
-        """Advances every instance by up to ``steps`` time steps; returns the
-        ids whose results are complete."""
+        """Adds the requests prepared since the last call, advances every
+        instance by up to ``steps`` time steps and returns the ids whose
+        results are complete (or whose preparation failed)."""
+        ready = []
+        while self._prepared:
+            pid, prep = self._prepared.popleft()
+            self._pending -= 1
+            if isinstance(prep, BaseException):
+                self._done[pid] = prep
+                ready.append(pid)
+                continue
+            try:
+                self._commit(pid, prep)
+            except NativeLibraryError as exc:
+                self._done[pid] = exc
+                ready.append(pid)
         n = self._lib.dgx_batch_step(self._handle, int(steps))
         if n < 0:
             raise NativeLibraryError(self._error())
-        if steps > 0:
+        if steps > 0 and self.active:
             # bound the queued work so that an instance added next joins soon
             mark = torch.cuda.Event()
             mark.record(self.stream)
             self._marks.append(mark)
             while len(self._marks) > self.lookahead:
                 self._marks.popleft().synchronize()
-        ready = []
-        for nid in list(self._running):
+        for pid in list(self._running):
+            nid, inst = self._running[pid]
             state = self._lib.dgx_batch_poll(self._handle, nid)
             if state < 0:
                 raise NativeLibraryError(self._error())
             if state == 1:
-                self._done[nid] = self._running.pop(nid)
-                ready.append(nid)
+                del self._running[pid]
+                self._done[pid] = inst
+                ready.append(pid)
         return ready
 
     def synchronize(self) -> None:
@@ -160,19 +233,24 @@ class BatchSolver:
         return int(self._lib.dgx_batch_active(self._handle))
 
     def run(self, chunk: int = 64) -> List[int]:
-        """Steps until every instance has finished; returns the finished ids."""
+        """Steps until every instance has finished and every submitted request
+        has been added; returns the finished ids."""
         ready = []
-        while self._running:
+        while self._running or self._pending:
             ready += self.step(chunk if self.active else 0)
-            if not self.active and self._running:
-                self.synchronize()
+            if self._running and not self.active:
+                self.synchronize()  # only completion events are left
+            elif self._pending and not self._prepared and not self.active:
+                time.sleep(0.0005)  # the preparation thread is working
         return ready
 
-    def result(self, nid: int):
+    def result(self, pid: int):
         """(receivers, grad_eps_r, grad_sigma) of a finished instance, on the
         CPU; gradients are folded onto the physical model (None for a
-        forward-only instance)."""
-        inst = self._done.pop(nid)  # its completion event has passed
+        forward-only instance). Raises what ended a failed one."""
+        inst = self._done.pop(pid)  # its completion event has passed
+        if isinstance(inst, BaseException):
+            raise inst
         with torch.cuda.device(self.device), torch.cuda.stream(self.side):
             g_eps = g_sig = None
             if inst.gradient:
@@ -183,6 +261,10 @@ class BatchSolver:
         return rx, g_eps, g_sig
 
     def close(self) -> None:
+        if self._worker is not None:
+            self._inbox.put(None)
+            self._worker.join()
+            self._worker = None
         if self._handle:
             self._lib.dgx_batch_destroy(self._handle)
             self._handle = None
@@ -214,11 +296,9 @@ def _main() -> int:
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     with BatchSolver(dev) as solver:
-        # longest first, and the device steps while the host prepares the next
-        ids = {}
-        for r in sorted(reqs, key=lambda q: (2 if q.gradient else 1) * q.nt, reverse=True):
-            ids[solver.add(r)] = r.rid
-            solver.step(args.chunk)
+        # longest first; a background thread prepares requests while the batch steps
+        ids = {solver.submit(r): r.rid for r in sorted(reqs, key=lambda q: (2 if q.gradient else 1) * q.nt,
+                                                         reverse=True)}
         solver.run(args.chunk)
         results = {ids[n]: solver.result(n) for n in ids}
     torch.cuda.synchronize(dev)

@@ -1332,14 +1332,25 @@ struct Staging {
     hipEvent_t done = nullptr;
 };
 
+// Device memory the batch stream may still read, kept until an event
+// recorded after its last reader has completed. Only then is it freed, so a
+// later stream-ordered allocation on any stream (instances are set up on
+// another stream) can never receive memory a queued step still uses.
+struct Retired {
+    std::unique_ptr<BatchInstance> inst;
+    std::unique_ptr<DevBuf> table;
+    hipEvent_t done = nullptr;
+};
+
 struct Batch {
     hipStream_t stream = nullptr;
     int gstep = 0;
     int next_id = 1;
     std::vector<std::unique_ptr<BatchInstance>> active;
     std::vector<std::pair<int, hipEvent_t>> finished;
+    std::vector<Retired> retired;
     bool dirty = true;
-    DevBuf table;  // descriptors and prefixes of the 6 slots
+    std::unique_ptr<DevBuf> table;  // descriptors and prefixes of the 6 slots
     size_t table_cap = 0;
     Staging ring[8];
     int ring_next = 0;
@@ -1348,6 +1359,8 @@ struct Batch {
     int ninst[6] = {}, total[6] = {}, order_of[6] = {2, 4, 8, 2, 4, 8};
     ~Batch()
     {
+        for (Retired& r : retired)
+            if (r.done) (void)hipEventDestroy(r.done);
         for (Staging& r : ring) {
             if (r.done) (void)hipEventDestroy(r.done);
             if (r.host) (void)hipHostFree(r.host);
@@ -1357,12 +1370,40 @@ struct Batch {
 
 int order_slot(int order) { return order == 8 ? 2 : (order == 4 ? 1 : 0); }
 
+// Moves memory the batch stream may still read to the retired list, behind
+// an event recorded on the batch stream now (after every launch queued so far).
+int batch_retire(Batch* b, std::unique_ptr<BatchInstance> inst, std::unique_ptr<DevBuf> table)
+{
+    Retired r;
+    r.inst = std::move(inst);
+    r.table = std::move(table);
+    DGX_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming), "batch: event");
+    DGX_TRY(hipEventRecord(r.done, b->stream), "batch: event");
+    b->retired.push_back(std::move(r));
+    return 0;
+}
+
+// Frees retired memory whose last reader has finished (no waiting).
+void batch_sweep(Batch* b)
+{
+    for (size_t q = 0; q < b->retired.size();) {
+        Retired& r = b->retired[q];
+        if (hipEventQuery(r.done) != hipSuccess) {
+            ++q;
+            continue;
+        }
+        (void)hipEventDestroy(r.done);
+        b->retired.erase(b->retired.begin() + (long)q);  // frees the buffers
+    }
+}
+
 // Sets an instance up on `stream` (not the batch stream, so that the host
 // waits only for the set-up work); the batch stream waits for it before the
 // instance's first step.
 int batch_add(Batch* b, const dgx_solve* P, const dgx_solve* G, hipStream_t stream)
 {
     CallStream call(stream);
+    batch_sweep(b);
     Plan pl;
     if (make_plan(P, false, pl)) return -1;
     const Grid& g = pl.g;
@@ -1524,9 +1565,11 @@ int batch_rebuild(Batch* b)
         bytes = align(bytes + prefix[slot].size() * sizeof(int));
     }
     if (bytes > b->table_cap) {
-        b->table.reset();  // stream-ordered: earlier steps finish reading it first
+        // queued steps still read the old table: retire it behind an event
+        if (b->table && batch_retire(b, nullptr, std::move(b->table))) return -1;
         const size_t cap = std::max(bytes, 2 * b->table_cap);
-        if (!b->table.alloc(cap)) return -1;
+        b->table = std::make_unique<DevBuf>();
+        if (!b->table->alloc(cap)) return -1;
         b->table_cap = cap;
     }
     Staging& r = b->ring[b->ring_next];
@@ -1544,7 +1587,7 @@ int batch_rebuild(Batch* b)
     }
     if (!r.done && hipEventCreateWithFlags(&r.done, hipEventDisableTiming) != hipSuccess) return -1;
     char* h = (char*)r.host;
-    char* dv = (char*)b->table.p;
+    char* dv = (char*)b->table->p;
     for (int slot = 0; slot < 6; ++slot) {
         std::memcpy(h + off_desc[slot], desc[slot].data(), desc[slot].size() * sizeof(TmBatchInst));
         std::memcpy(h + off_prefix[slot], prefix[slot].data(), prefix[slot].size() * sizeof(int));
@@ -1636,9 +1679,12 @@ int batch_step(Batch* b, int steps)
             DGX_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "batch: event");
             DGX_TRY(hipEventRecord(ev, b->stream), "batch: event");
             b->finished.push_back({I.id, ev});
-            b->active.erase(b->active.begin() + (long)q);  // scratch is freed stream-ordered
+            std::unique_ptr<BatchInstance> done = std::move(b->active[q]);
+            b->active.erase(b->active.begin() + (long)q);
+            if (batch_retire(b, std::move(done), nullptr)) return -1;
         }
     }
+    batch_sweep(b);
     DGX_TRY(hipGetLastError(), "batch: kernel launch");
     return (int)b->active.size();
 }
@@ -1723,8 +1769,9 @@ DGX_API void dgx_batch_destroy(void* batch)
     {
         CallStream call(b->stream);
         b->active.clear();
+        b->table.reset();
     }
-    delete b;
+    delete b;  // the retired memory too: every event has completed
 }
 
 DGX_API int dgx_codec_roundtrip(const float* input, void* encoded, float* decoded, long long count,

@@ -275,16 +275,16 @@ class SolveServer:
                         if not self._inbox and not live:
                             return
                         new, self._inbox = self._inbox, []
-                    for item in new:
-                        req, fut, need, submitted = item
-                        try:
-                            live[solver.add(req)] = (item, time.perf_counter())
-                        except BaseException as exc:  # noqa: BLE001 - delivered to the caller
-                            fut.set_exception(exc)
-                            self._release(need, True)
+                    for item in new:  # prepared on the solver's thread while the batch steps
+                        live[solver.submit(item[0])] = (item, time.perf_counter())
                     ready = solver.step(self.batch_chunk if solver.active else 0)
                     if not ready and live and not solver.active:
-                        solver.synchronize()  # only completion events are left
+                        if solver.pending:  # only preparations are left: wait for them or new work
+                            with self._cv:
+                                if not self._inbox:
+                                    self._cv.wait(timeout=0.0005)
+                        else:
+                            solver.synchronize()  # only completion events are left
                     for nid in ready:
                         (req, fut, need, submitted), started = live.pop(nid)
                         try:

@@ -12,7 +12,8 @@ inputs.
 from __future__ import annotations
 
 import math
-from typing import Any, List, Optional, Sequence, Tuple
+import threading
+from typing import Any, Dict, List, Optional, Sequence, Tuple
 
 import torch
 
@@ -100,6 +101,46 @@ def pml_face_descriptors(nx: int, ny: int, nz: int, pmlthick: Any) -> Tuple[torc
                  else torch.empty(0) for f in range(CPML_FACE_COUNT))
 
 
+_PROFILE_CACHE: Dict[tuple, tuple] = {}
+_PROFILE_LOCK = threading.Lock()
+
+
+def _face_profiles(p: int, dt: float, d: float, v_ref: float, device, dtype):
+    """(electric, magnetic) coefficients ``(4, 1, p)`` of one face. They depend
+    only on these arguments, so they are computed once (with the same float32
+    operations, on the device) and copied for every later call; a copy on
+    another stream waits for the computation through an event."""
+    dev = torch.device(device)
+    key = (p, float(dt), float(d), v_ref, str(dev), dtype)
+    with _PROFILE_LOCK:
+        hit = _PROFILE_CACHE.get(key)
+    if hit is None:
+        cfs = CFS(device=dev)
+        order = CFSParameter.scalingprofiles[cfs.sigma.scalingprofile]
+        sigma_max = CPML_SIGMA_MAX_FACTOR * (order + 1) * e0 * v_ref / d
+        finfo = torch.finfo(torch.get_default_dtype())
+        if not math.isfinite(sigma_max) or not finfo.tiny <= sigma_max <= finfo.max:
+            raise ValueError("max_velocity produces an unrepresentable PML absorption profile.")
+        cfs.sigma.max = sigma_max
+        shape = (CPML_COEFFICIENTS_PER_CELL, CPML_POLE_COUNT, p)
+        el = torch.zeros(shape, device=dev, dtype=dtype)
+        ma = torch.zeros(shape, device=dev, dtype=dtype)
+        calculate_pml_update_coeffs(cfs, el, ma, 1.0, 1.0, dt, d, p)
+        if not bool(torch.isfinite(el).all() & torch.isfinite(ma).all()):
+            raise ValueError("max_velocity produces non-finite PML update coefficients.")
+        ready = None
+        if dev.type == "cuda":
+            ready = torch.cuda.Event()
+            ready.record(torch.cuda.current_stream(dev))
+        hit = (el, ma, ready)
+        with _PROFILE_LOCK:
+            _PROFILE_CACHE.setdefault(key, hit)
+    el, ma, ready = hit
+    if ready is not None:
+        torch.cuda.current_stream(dev).wait_event(ready)
+    return el.clone(), ma.clone()
+
+
 def build_pml_coeffs(eps_r, mu_r, dt, dx, nx, ny, nz, pmlthick, device, dtype, *,
                      max_velocity=SPEED_OF_LIGHT):
     """Six face descriptors followed by (electric, magnetic) coefficients per
@@ -121,20 +162,7 @@ def build_pml_coeffs(eps_r, mu_r, dt, dx, nx, ny, nz, pmlthick, device, dtype, *
             coefficients.extend((torch.empty(0), torch.empty(0)))
             continue
         descriptors.append(torch.tensor(_face_box(face, p, nx, ny, nz), dtype=torch.int))
-        cfs = CFS(device=device)
-        order = CFSParameter.scalingprofiles[cfs.sigma.scalingprofile]
-        sigma_max = CPML_SIGMA_MAX_FACTOR * (order + 1) * e0 * v_ref / spacing[face // 2]
-        finfo = torch.finfo(torch.get_default_dtype())
-        if not math.isfinite(sigma_max) or not finfo.tiny <= sigma_max <= finfo.max:
-            raise ValueError("max_velocity produces an unrepresentable PML absorption profile.")
-        cfs.sigma.max = sigma_max
-        shape = (CPML_COEFFICIENTS_PER_CELL, CPML_POLE_COUNT, p)
-        el = torch.zeros(shape, device=device, dtype=dtype)
-        ma = torch.zeros(shape, device=device, dtype=dtype)
-        calculate_pml_update_coeffs(cfs, el, ma, 1.0, 1.0, dt, spacing[face // 2], p)
-        if not bool(torch.isfinite(el).all() & torch.isfinite(ma).all()):
-            raise ValueError("max_velocity produces non-finite PML update coefficients.")
-        coefficients.extend((el, ma))
+        coefficients.extend(_face_profiles(p, dt, spacing[face // 2], v_ref, device, dtype))
     return (*descriptors, *coefficients)
 
 

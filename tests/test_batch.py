@@ -36,9 +36,10 @@ def test_batch_matches_compute(cuda_device):
     reqs[-1].fdtd_order = 8
     alone = serving.run_sequential(reqs, cuda_device)["results"]
     with batch.BatchSolver(cuda_device) as solver:
-        ids = {solver.add(r): r.rid for r in reqs[:-2]}
-        solver.step(7)  # the last two join a running batch
-        ids.update({solver.add(r): r.rid for r in reqs[-2:]})
+        ids = {solver.add(r): r.rid for r in reqs[:-4]}
+        solver.step(7)  # the others join a running batch
+        ids.update({solver.add(r): r.rid for r in reqs[-4:-2]})
+        ids.update({solver.submit(r): r.rid for r in reqs[-2:]})  # prepared on the background thread
         solver.run(16)
         got = {ids[n]: solver.result(n) for n in ids}
     for r in reqs:

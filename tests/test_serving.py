@@ -54,13 +54,17 @@ class _FakeBatch:
     def __init__(self, device, fail_rid=None):
         self.left, self.reqs, self.next, self.peak, self.closed = {}, {}, 0, 0, False
         self.fail_rid = fail_rid
+        self.failed = set()
         self.seen = []
         _FakeBatch.made.append(self)
 
-    def add(self, req):
-        if req.rid == self.fail_rid:
-            raise ValueError("cannot batch this one")
+    pending = 0
+
+    def submit(self, req):
         nid, self.next = self.next, self.next + 1
+        if req.rid == self.fail_rid:
+            self.failed.add(nid)
+            return nid
         self.left[nid], self.reqs[nid] = req.nt, req
         self.peak = max(self.peak, len(self.left))
         self.seen.append(req.rid)
@@ -68,7 +72,9 @@ class _FakeBatch:
 
     def step(self, steps):
         time.sleep(0.002 if steps else 0.0)  # a group of steps takes device time
-        ready = [n for n in self.left if self.left[n] <= steps]
+        ready = sorted(self.failed)
+        self.failed.clear()
+        ready += [n for n in self.left if self.left[n] <= steps]
         for n in list(self.left):
             self.left[n] -= steps
             if self.left[n] <= 0:
@@ -83,6 +89,8 @@ class _FakeBatch:
         pass
 
     def result(self, nid):
+        if nid not in self.reqs:
+            raise ValueError("cannot batch this one")
         req = self.reqs.pop(nid)
         return torch.full((1, req.nt, 2), float(req.rid)), None, None
 
