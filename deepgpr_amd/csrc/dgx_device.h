@@ -304,6 +304,26 @@ __device__ __forceinline__ void psi_dims(const Grid& g, bool electric, int face,
     }
 }
 
+// Whether cell c has an element in auxiliary array w of face f (inside the
+// face region along the face axis, inside the array's shape across it).
+__device__ __forceinline__ bool psi_active(const Grid& g, bool electric, int face, int w, const int c[3])
+{
+    int dims[3];
+    psi_dims(g, electric, face, w, dims);
+    const int axis = face >> 1;
+    const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
+    return c[a1] < dims[a1] && c[a2] < dims[a2];
+}
+
+// The six staggered differences of a curl, (component, axis) per slot:
+// z/y, y/z (first component), x/z, z/x (second), y/x, x/y (third).
+constexpr int kDiffComp[6] = {2, 1, 0, 2, 1, 0};
+constexpr int kDiffAxis[6] = {1, 2, 2, 0, 0, 1};
+__device__ __forceinline__ int diff_slot(int comp, int axis)
+{
+    return axis == 0 ? (comp == 2 ? 3 : 4) : (axis == 1 ? (comp == 2 ? 0 : 5) : (comp == 1 ? 1 : 2));
+}
+
 // Profile index of coordinate c on a face, or -1 outside the face region.
 __device__ __forceinline__ int face_q(const Grid& g, bool electric, int face, int c)
 {

@@ -92,21 +92,27 @@ __global__ __launch_bounds__(256) void k_fwd_h(Grid g, const float* __restrict__
     float h[3] = {H.c[0][n], H.c[1][n], H.c[2][n]};
     const float u1 = hcurl[m];
     const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
-    if (up0) {
-        float dEz = dfwd<ORDER>(E.c[2], n, st[1], j, g.N[1]);
-        float dEy = dfwd<ORDER>(E.c[1], n, 1, k, g.N[2]);
-        h[0] = h[0] - uy * dEz + uz * dEy;
+    // Each staggered difference is evaluated once and shared by the curl
+    // update and the CPML terms that use it (the same call either way).
+    bool need[6] = {up0, up0, up1, up1, up2, up2};
+    if (pml) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const int axis = f >> 1;
+            if (face_q(g, false, f, c[axis]) < 0) continue;
+#pragma unroll
+            for (int w = 0; w < 2; ++w)
+                if (psi_active(g, false, f, w, c)) need[diff_slot(face_source(axis, w), axis)] = true;
+        }
     }
-    if (up1) {
-        float dEx = dfwd<ORDER>(E.c[0], n, 1, k, g.N[2]);
-        float dEz = dfwd<ORDER>(E.c[2], n, st[0], i, g.N[0]);
-        h[1] = h[1] - uz * dEx + u1 * dEz;
-    }
-    if (up2) {
-        float dEy = dfwd<ORDER>(E.c[1], n, st[0], i, g.N[0]);
-        float dEx = dfwd<ORDER>(E.c[0], n, st[1], j, g.N[1]);
-        h[2] = h[2] - u1 * dEy + uy * dEx;
-    }
+    float dv[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d)
+        dv[d] = need[d] ? dfwd<ORDER>(E.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]], g.N[kDiffAxis[d]])
+                        : 0.0f;
+    if (up0) h[0] = h[0] - uy * dv[0] + uz * dv[1];  // dEz/dy, dEy/dz
+    if (up1) h[1] = h[1] - uz * dv[2] + u1 * dv[3];  // dEx/dz, dEz/dx
+    if (up2) h[2] = h[2] - u1 * dv[4] + uy * dv[5];  // dEy/dx, dEx/dy
     if (pml) {
         const float upd = hrhs[m];
 #pragma unroll
@@ -119,12 +125,11 @@ __global__ __launch_bounds__(256) void k_fwd_h(Grid g, const float* __restrict__
             const float ra1 = R[q] - 1.0f, rb = R[pp + q], re = R[2 * pp + q], rf = R[3 * pp + q];
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
+                if (!psi_active(g, false, f, w, c)) continue;
                 int dims[3];
                 psi_dims(g, false, f, w, dims);
-                const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
-                if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
                 const int t = face_target(axis, w), sc = face_source(axis, w);
-                const float der = dfwd<ORDER>(E.c[sc], n, st[axis], c[axis], g.N[axis]) / g.d[axis];
+                const float der = dv[diff_slot(sc, axis)] / g.d[axis];
                 float* psi = cp.psi[f][w];
                 const long long pi = psi_index(dims, s, c, axis, q);
                 const float phi = psi[pi];
@@ -196,21 +201,27 @@ __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__
             e[2] = u0 * e[2] + u1 * dHy - uy * dHx;
         }
     } else {
-        if (up[0]) {
-            float dHz = dbwd<ORDER>(Hf.c[2], n, st[1], j, g.N[1]);
-            float dHy = dbwd<ORDER>(Hf.c[1], n, 1, k, g.N[2]);
-            e[0] = u0 * e[0] + uy * dHz - uz * dHy;
+        // each staggered difference once, shared by the curl update and the CPML terms
+        bool need[6] = {up[0], up[0], up[1], up[1], up[2], up[2]};
+        if (pml) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                const int axis = f >> 1;
+                if (face_q(g, true, f, c[axis]) < 0) continue;
+#pragma unroll
+                for (int w = 0; w < 2; ++w)
+                    if (psi_active(g, true, f, w, c)) need[diff_slot(face_source(axis, w), axis)] = true;
+            }
         }
-        if (up[1]) {
-            float dHx = dbwd<ORDER>(Hf.c[0], n, 1, k, g.N[2]);
-            float dHz = dbwd<ORDER>(Hf.c[2], n, st[0], i, g.N[0]);
-            e[1] = u0 * e[1] + uz * dHx - u1 * dHz;
-        }
-        if (up[2]) {
-            float dHy = dbwd<ORDER>(Hf.c[1], n, st[0], i, g.N[0]);
-            float dHx = dbwd<ORDER>(Hf.c[0], n, st[1], j, g.N[1]);
-            e[2] = u0 * e[2] + u1 * dHy - uy * dHx;
-        }
+        float dv[6];
+#pragma unroll
+        for (int d = 0; d < 6; ++d)
+            dv[d] = need[d] ? dbwd<ORDER>(Hf.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]],
+                                          g.N[kDiffAxis[d]])
+                            : 0.0f;
+        if (up[0]) e[0] = u0 * e[0] + uy * dv[0] - uz * dv[1];  // dHz/dy, dHy/dz
+        if (up[1]) e[1] = u0 * e[1] + uz * dv[2] - u1 * dv[3];  // dHx/dz, dHz/dx
+        if (up[2]) e[2] = u0 * e[2] + u1 * dv[4] - uy * dv[5];  // dHy/dx, dHx/dy
         if (pml) {
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
@@ -222,12 +233,11 @@ __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__
                 const float ra1 = R[q] - 1.0f, rb = R[pp + q], re = R[2 * pp + q], rf = R[3 * pp + q];
 #pragma unroll
                 for (int w = 0; w < 2; ++w) {
+                    if (!psi_active(g, true, f, w, c)) continue;
                     int dims[3];
                     psi_dims(g, true, f, w, dims);
-                    const int a1 = axis == 2 ? 0 : axis + 1, a2 = axis == 0 ? 2 : axis - 1;
-                    if (c[a1] >= dims[a1] || c[a2] >= dims[a2]) continue;
                     const int t = face_target(axis, w), sc = face_source(axis, w);
-                    const float der = dbwd<ORDER>(Hf.c[sc], n, st[axis], c[axis], g.N[axis]) / g.d[axis];
+                    const float der = dv[diff_slot(sc, axis)] / g.d[axis];
                     float* psi = cp.psi[f][w];
                     const long long pi = psi_index(dims, s, c, axis, q);
                     const float phi = psi[pi];
