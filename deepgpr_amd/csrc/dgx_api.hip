@@ -690,7 +690,7 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     float* const* co = P->update_coeff;
 
     // --- sources: group entries by the block of the electric sweep ---------
-    dim3 grid = cell_grid(g);
+    const CellSweep sweep = cell_sweep(g, pl.order);
     std::vector<int> hsloc((size_t)g.nshot * std::max(P->nsource, 0) * 3);
     if (P->nsource > 0) {
         if (!P->source_location || !P->source_waveform) {
@@ -730,7 +730,7 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     const bool fused = tm && tm2d_fused();
     const Tm2dGrid tg = tm_grid_of(pl);
     const int tm_tiles = (g.N[0] * g.N[1] + 255) / 256;
-    long long nblocks = tm ? (long long)tm_tiles * g.nshot : (long long)grid.x * grid.y * grid.z;
+    long long nblocks = tm ? (long long)tm_tiles * g.nshot : sweep.blocks;
     std::vector<int> boff((size_t)nblocks + 1, 0);
     std::vector<int2> ents;
     if (inject && fused) {
@@ -747,8 +747,11 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
             for (int k = 0; k < P->nsource; ++k) {
                 const int* L = &hsloc[((size_t)s * P->nsource + k) * 3];
                 long long m = (long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2];
-                int p = L[1] * g.N[2] + L[2];
-                long long blk = (long long)(p / 256) + (long long)grid.x * (L[0] + (long long)grid.y * s);
+                long long blk = cell_block(sweep, g, L[0], L[1], L[2], s);
+                if (blk < 0 && !tm) {
+                    set_error("forward: source cell outside the cell sweep");
+                    return -1;
+                }
                 if (tm) {  // compact plane index and 2D tile
                     m = (long long)L[0] * g.N[1] + L[1];
                     blk = (long long)(m / 256) + (long long)tm_tiles * s;
@@ -1225,9 +1228,9 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             TIMED(tmr, kKAdjH, launch_tm_adj_h(pl.order, tms.g, tms.f, tw, stream));
         } else {
             float* W = (float*)d_w.p;
-            TIMED(tmr, kKAdjWeightsE, launch_adj_weights(true, g, a, W, stream));
+            TIMED(tmr, kKAdjWeightsE, launch_adj_weights(pl.order, true, g, a, W, stream));
             TIMED(tmr, kKAdjE, launch_adj_gather_e(pl.order, g, a, W, stream));
-            TIMED(tmr, kKAdjWeightsH, launch_adj_weights(false, g, a, W, stream));
+            TIMED(tmr, kKAdjWeightsH, launch_adj_weights(pl.order, false, g, a, W, stream));
             TIMED(tmr, kKAdjH, launch_adj_gather_h(pl.order, g, a, W, stream));
         }
     }
@@ -1798,8 +1801,7 @@ DGX_API long long dgx_scratch_bytes(const dgx_solve* solve, int backward)
     long long bytes = 0;
     if (!backward) {
         if (pl.history && pl.store_r && pl.kind == kInt8) bytes += (long long)pl.comps * pl.snap * 4;
-        dim3 grid = cell_grid(g);
-        bytes += ((long long)grid.x * grid.y * grid.z + 1) * 4;
+        bytes += (cell_sweep(g, pl.order).blocks + 1) * 4;
     } else {
         if (pl.has_cpml)
             for (int f = 0; f < 6; ++f)

@@ -324,6 +324,25 @@ __device__ __forceinline__ int diff_slot(int comp, int axis)
     return axis == 0 ? (comp == 2 ? 3 : 4) : (axis == 1 ? (comp == 2 ? 0 : 5) : (comp == 1 ? 1 : 2));
 }
 
+// Cells of one launch of a cell sweep: every row j of the plane, and in each
+// row the k of two runs [za, za+na) and [zb, zb+nb). base numbers the
+// launch's workgroups after those of the launches before it.
+struct ZRuns {
+    int za, na, zb, nb;
+    int base;
+};
+
+__device__ __forceinline__ bool run_cell(const Grid& g, const ZRuns& zr, int& j, int& k)
+{
+    const int per = zr.na + zr.nb;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= per * g.N[1]) return false;
+    j = q / per;
+    const int r = q - j * per;
+    k = r < zr.na ? zr.za + r : zr.zb + (r - zr.na);
+    return true;
+}
+
 // Profile index of coordinate c on a face, or -1 outside the face region.
 __device__ __forceinline__ int face_q(const Grid& g, bool electric, int face, int c)
 {

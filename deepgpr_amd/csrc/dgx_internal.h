@@ -245,7 +245,16 @@ void launch_tm_adj_h(int order, const Tm2dGrid& g, const Tm2dFields& f, const Tm
 void launch_tm_material_gradient(const Tm2dGrid& g, const Tm2dFields& f, const MatGrad& mg, int ez_comp,
                                  void* geps, void* gsig, bool fp64, hipStream_t st);
 
-dim3 cell_grid(const Grid& g);
+// The launches of a one-thread-per-cell sweep (see cell_sweep).
+struct CellSweep {
+    ZRuns run[2];
+    dim3 grid[2];
+    int count;
+    long long blocks;  // workgroups of all launches
+};
+CellSweep cell_sweep(const Grid& g, int order, bool split = true);
+// Workgroup (numbered across the launches) that advances cell (i, j, k) of shot s.
+long long cell_block(const CellSweep& cs, const Grid& g, int i, int j, int k, int s);
 void launch_coefficients(const Grid& g, const float* eps, const float* sig, const float* mu, float* const co[6],
                          float dt, hipStream_t st);
 void launch_fwd_h(int order, const Grid& g, const StepArgs& a, hipStream_t st);
@@ -264,7 +273,7 @@ void launch_material_gradient(const Grid& g, CF3 L, const MatGrad& mg, void* gep
                               hipStream_t st);
 void launch_add_f64(const double* a, float* out, long long count, hipStream_t st);
 // two-pass adjoint: per-cell channel weights (6 x nshot x cells floats), then gathers
-void launch_adj_weights(bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st);
+void launch_adj_weights(int order, bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st);
 void launch_adj_gather_e(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st);
 void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st);
 

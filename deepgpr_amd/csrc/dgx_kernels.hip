@@ -5,6 +5,8 @@
 // electric sweep. The adjoint is a deterministic gather: every target cell
 // re-evaluates the transposed stencil weights of its neighbours, so no
 // atomics are used anywhere and results do not depend on scheduling.
+#include <algorithm>
+
 #include "dgx_device.h"
 #include "dgx_internal.h"
 
@@ -49,13 +51,12 @@ __global__ __launch_bounds__(256) void k_coefficients(
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_fwd_h(Grid g, const float* __restrict__ hcurl,
                                                const float* __restrict__ hrhs, CF3 E, F3 H, Cpml cp,
-                                               int has_cpml)
+                                               int has_cpml, ZRuns zr)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.plane) return;
+    int j, k;
+    if (!run_cell(g, zr, j, k)) return;
+    const int p = j * g.N[2] + k;
     const int i = blockIdx.y, s = blockIdx.z;
-    const int j = p / g.N[2];
-    const int k = p - j * g.N[2];
     const int c[3] = {i, j, k};
     const long long m = (long long)i * g.st0 + p;
     const long long n = (long long)s * g.cells + m;
@@ -153,13 +154,12 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__ ca,
                                                const float* __restrict__ ccurl,
                                                const float* __restrict__ cb, CF3 Hf, F3 E, Cpml cp,
-                                               int has_cpml, SourceBlocks src, HistWrite hist)
+                                               int has_cpml, SourceBlocks src, HistWrite hist, ZRuns zr)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.plane) return;
+    int j, k;
+    if (!run_cell(g, zr, j, k)) return;
+    const int p = j * g.N[2] + k;
     const int i = blockIdx.y, s = blockIdx.z;
-    const int j = p / g.N[2];
-    const int k = p - j * g.N[2];
     const int c[3] = {i, j, k};
     const long long m = (long long)i * g.st0 + p;
     const long long n = (long long)s * g.cells + m;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__
     const bool pml = has_cpml && in_any_face(g, true, c);
 
     // block-local source entries (sorted by block, then source order)
-    const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int blin = zr.base + blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     int sb = 0, se = 0;
     if (src.count > 0) {
         sb = src.block_off[blin];
@@ -470,13 +470,12 @@ __device__ __forceinline__ int chan_of(int u, int a) { return 2 * u + (a > u ? a
 template <bool ELECTRIC>
 __global__ __launch_bounds__(256) void k_adj_weights(Grid g, const float* __restrict__ curl,
                                                      const float* __restrict__ upd, CF3 lam, Cpml cp,
-                                                     float* __restrict__ W, int has_cpml)
+                                                     float* __restrict__ W, int has_cpml, ZRuns zr)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.plane) return;
+    int j, k;
+    if (!run_cell(g, zr, j, k)) return;
+    const int p = j * g.N[2] + k;
     const int i = blockIdx.y, s = blockIdx.z;
-    const int j = p / g.N[2];
-    const int k = p - j * g.N[2];
     const int c[3] = {i, j, k};
     const long long m = (long long)i * g.st0 + p;
     const long long n = (long long)s * g.cells + m;
@@ -546,13 +545,12 @@ __global__ __launch_bounds__(256) void k_adj_weights(Grid g, const float* __rest
 
 // lambda_H += D-^T W_E (the E-side weights); lambda_E is scaled later.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_adj_gather_e(Grid g, const float* __restrict__ W, F3 lamH)
+__global__ __launch_bounds__(256) void k_adj_gather_e(Grid g, const float* __restrict__ W, F3 lamH, ZRuns zr)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.plane) return;
+    int j, k;
+    if (!run_cell(g, zr, j, k)) return;
+    const int p = j * g.N[2] + k;
     const int i = blockIdx.y, s = blockIdx.z;
-    const int j = p / g.N[2];
-    const int k = p - j * g.N[2];
     const int c[3] = {i, j, k};
     const long long n = (long long)s * g.cells + (long long)i * g.st0 + p;
     const long long st[3] = {g.st0, (long long)g.N[2], 1};
@@ -574,13 +572,12 @@ __global__ __launch_bounds__(256) void k_adj_gather_e(Grid g, const float* __res
 // lambda_E = ca lambda_E (updated components) + D+^T W_H.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_adj_gather_h(Grid g, const float* __restrict__ ca,
-                                                      const float* __restrict__ W, F3 lamE)
+                                                      const float* __restrict__ W, F3 lamE, ZRuns zr)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.plane) return;
+    int j, k;
+    if (!run_cell(g, zr, j, k)) return;
+    const int p = j * g.N[2] + k;
     const int i = blockIdx.y, s = blockIdx.z;
-    const int j = p / g.N[2];
-    const int k = p - j * g.N[2];
     const int c[3] = {i, j, k};
     const long long m = (long long)i * g.st0 + p;
     const long long n = (long long)s * g.cells + m;
@@ -606,13 +603,51 @@ __global__ __launch_bounds__(256) void k_adj_gather_h(Grid g, const float* __res
 // Launch helpers
 // ---------------------------------------------------------------------------
 
-dim3 cell_grid(const Grid& g)
+CellSweep cell_sweep(const Grid& g, int order, bool split)
 {
-    unsigned bx = (unsigned)((g.plane + 255) / 256);
-    // keep the x-planes of one column of blocks on the same XCD (8 XCDs,
-    // round-robin block placement) so x-neighbours are served by its L2
-    if (bx >= 8) bx = (bx + 7) / 8 * 8;
-    return dim3(bx, (unsigned)g.N[0], (unsigned)g.nshot);
+    // The z-interior of every row (no z-face CPML term, no clamped z
+    // difference) and the rest run as two launches, so that a wavefront of
+    // consecutive cells rarely mixes the interior and the CPML branches.
+    const int R = order / 2, nz = g.N[2];
+    int lo = std::max(R, g.pml[4] > 0 ? g.pml[4] + 1 : 0);
+    int hi = nz - 1 - R;
+    if (g.pml[5] > 0) hi = std::min(hi, nz - 2 - g.pml[5]);
+    CellSweep cs{};
+    if (!split || lo > hi || hi - lo + 1 < 64) {
+        cs.count = 1;
+        cs.run[0] = ZRuns{0, nz, 0, 0, 0};
+    } else {
+        cs.count = 2;
+        cs.run[0] = ZRuns{lo, hi - lo + 1, 0, 0, 0};
+        cs.run[1] = ZRuns{0, lo, hi + 1, nz - 1 - hi, 0};
+    }
+    long long base = 0;
+    for (int q = 0; q < cs.count; ++q) {
+        const long long cells = (long long)(cs.run[q].na + cs.run[q].nb) * g.N[1];
+        unsigned bx = (unsigned)((cells + 255) / 256);
+        // keep the x-planes of one column of blocks on the same XCD (8 XCDs,
+        // round-robin block placement) so x-neighbours are served by its L2
+        if (bx >= 8) bx = (bx + 7) / 8 * 8;
+        cs.grid[q] = dim3(bx, (unsigned)g.N[0], (unsigned)g.nshot);
+        cs.run[q].base = (int)base;
+        base += (long long)bx * g.N[0] * g.nshot;
+    }
+    cs.blocks = base;
+    return cs;
+}
+
+long long cell_block(const CellSweep& cs, const Grid& g, int i, int j, int k, int s)
+{
+    for (int q = 0; q < cs.count; ++q) {
+        const ZRuns& z = cs.run[q];
+        int r = -1;
+        if (k >= z.za && k < z.za + z.na) r = k - z.za;
+        else if (k >= z.zb && k < z.zb + z.nb) r = z.na + (k - z.zb);
+        if (r < 0) continue;
+        const long long cell = (long long)j * (z.na + z.nb) + r;
+        return z.base + cell / 256 + (long long)cs.grid[q].x * (i + (long long)g.N[0] * s);
+    }
+    return -1;
 }
 
 void launch_coefficients(const Grid& g, const float* eps, const float* sig, const float* mu, float* const co[6],
@@ -625,27 +660,38 @@ void launch_coefficients(const Grid& g, const float* eps, const float* sig, cons
 
 void launch_fwd_h(int order, const Grid& g, const StepArgs& a, hipStream_t st)
 {
-    dim3 grid = cell_grid(g);
-    if (order == 8)
-        hipLaunchKernelGGL(k_fwd_h<8>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml);
-    else if (order == 4)
-        hipLaunchKernelGGL(k_fwd_h<4>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml);
-    else
-        hipLaunchKernelGGL(k_fwd_h<2>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml);
+    const CellSweep cs = cell_sweep(g, order);
+    for (int q = 0; q < cs.count; ++q) {
+        const dim3 grid = cs.grid[q];
+        const ZRuns zr = cs.run[q];
+        if (order == 8)
+            hipLaunchKernelGGL(k_fwd_h<8>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml,
+                               zr);
+        else if (order == 4)
+            hipLaunchKernelGGL(k_fwd_h<4>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml,
+                               zr);
+        else
+            hipLaunchKernelGGL(k_fwd_h<2>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml,
+                               zr);
+    }
 }
 
 void launch_fwd_e(int order, const Grid& g, const StepArgs& a, hipStream_t st)
 {
-    dim3 grid = cell_grid(g);
-    if (order == 8)
-        hipLaunchKernelGGL(k_fwd_e<8>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
-                           a.has_cpml, a.src, a.hist);
-    else if (order == 4)
-        hipLaunchKernelGGL(k_fwd_e<4>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
-                           a.has_cpml, a.src, a.hist);
-    else
-        hipLaunchKernelGGL(k_fwd_e<2>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
-                           a.has_cpml, a.src, a.hist);
+    const CellSweep cs = cell_sweep(g, order);
+    for (int q = 0; q < cs.count; ++q) {
+        const dim3 grid = cs.grid[q];
+        const ZRuns zr = cs.run[q];
+        if (order == 8)
+            hipLaunchKernelGGL(k_fwd_e<8>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
+                               a.has_cpml, a.src, a.hist, zr);
+        else if (order == 4)
+            hipLaunchKernelGGL(k_fwd_e<4>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
+                               a.has_cpml, a.src, a.hist, zr);
+        else
+            hipLaunchKernelGGL(k_fwd_e<2>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
+                               a.has_cpml, a.src, a.hist, zr);
+    }
 }
 
 void launch_sample(const Grid& g, const float* f, const int* loc, float* out, int nrx, int nt, int it,
@@ -710,31 +756,40 @@ void launch_add_f64(const double* a, float* out, long long count, hipStream_t st
     hipLaunchKernelGGL(k_add_f64_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, a, out, count);
 }
 
-void launch_adj_weights(bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st)
+void launch_adj_weights(int order, bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st)
 {
-    const dim3 grid = cell_grid(g);
-    if (electric)
-        hipLaunchKernelGGL(k_adj_weights<true>, grid, dim3(256), 0, st, g, a.ccurl, a.cb, a.lEc, a.cpe, W,
-                           a.has_cpml);
-    else
-        hipLaunchKernelGGL(k_adj_weights<false>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.lHc, a.cph, W,
-                           a.has_cpml);
+    const CellSweep cs = cell_sweep(g, order, false);  // measured: the split does not pay here
+    for (int q = 0; q < cs.count; ++q) {
+        if (electric)
+            hipLaunchKernelGGL(k_adj_weights<true>, cs.grid[q], dim3(256), 0, st, g, a.ccurl, a.cb, a.lEc, a.cpe, W,
+                               a.has_cpml, cs.run[q]);
+        else
+            hipLaunchKernelGGL(k_adj_weights<false>, cs.grid[q], dim3(256), 0, st, g, a.hcurl, a.hrhs, a.lHc, a.cph,
+                               W, a.has_cpml, cs.run[q]);
+    }
 }
 
 void launch_adj_gather_e(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st)
 {
-    const dim3 grid = cell_grid(g);
-    if (order == 8) hipLaunchKernelGGL(k_adj_gather_e<8>, grid, dim3(256), 0, st, g, W, a.lH);
-    else if (order == 4) hipLaunchKernelGGL(k_adj_gather_e<4>, grid, dim3(256), 0, st, g, W, a.lH);
-    else hipLaunchKernelGGL(k_adj_gather_e<2>, grid, dim3(256), 0, st, g, W, a.lH);
+    const CellSweep cs = cell_sweep(g, order);
+    for (int q = 0; q < cs.count; ++q) {
+        const dim3 grid = cs.grid[q];
+        if (order == 8) hipLaunchKernelGGL(k_adj_gather_e<8>, grid, dim3(256), 0, st, g, W, a.lH, cs.run[q]);
+        else if (order == 4) hipLaunchKernelGGL(k_adj_gather_e<4>, grid, dim3(256), 0, st, g, W, a.lH, cs.run[q]);
+        else hipLaunchKernelGGL(k_adj_gather_e<2>, grid, dim3(256), 0, st, g, W, a.lH, cs.run[q]);
+    }
 }
 
 void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st)
 {
-    const dim3 grid = cell_grid(g);
-    if (order == 8) hipLaunchKernelGGL(k_adj_gather_h<8>, grid, dim3(256), 0, st, g, a.ca, W, a.lE);
-    else if (order == 4) hipLaunchKernelGGL(k_adj_gather_h<4>, grid, dim3(256), 0, st, g, a.ca, W, a.lE);
-    else hipLaunchKernelGGL(k_adj_gather_h<2>, grid, dim3(256), 0, st, g, a.ca, W, a.lE);
+    const CellSweep cs = cell_sweep(g, order);
+    for (int q = 0; q < cs.count; ++q) {
+        const dim3 grid = cs.grid[q];
+        if (order == 8) hipLaunchKernelGGL(k_adj_gather_h<8>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.run[q]);
+        else if (order == 4)
+            hipLaunchKernelGGL(k_adj_gather_h<4>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.run[q]);
+        else hipLaunchKernelGGL(k_adj_gather_h<2>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.run[q]);
+    }
 }
 
 }  // namespace dgx

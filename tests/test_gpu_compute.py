@@ -100,7 +100,9 @@ def test_compute_c1_matches_oracle(oracle_built, cuda_device, monkeypatch, acc):
 
 
 # the larger grids have cells whose adjoint weights the gathers form themselves
-@pytest.mark.parametrize("shape,order", [((14, 12, 10), 4), ((12, 56, 44), 4), ((20, 40, 36), 8), ((16, 30, 26), 2)])
+# (11, 16, 90) and (12, 14, 96): rows long enough for the z-interior to run as its own launch
+@pytest.mark.parametrize("shape,order", [((14, 12, 10), 4), ((12, 56, 44), 4), ((20, 40, 36), 8), ((16, 30, 26), 2),
+                                         ((11, 16, 90), 4), ((12, 14, 96), 8)])
 def test_compute_3d_matches_oracle(oracle_built, cuda_device, monkeypatch, shape, order):
     import deepgpr_amd as dg
     h = oracle_built
