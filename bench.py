@@ -153,6 +153,9 @@ def kernel_bytes(kernel: str, w, args) -> float:
         return cells * 48               # six channel weights read, lambda_H read+write
     if kernel == "adj_h":
         return cells * 52               # six channel weights read, lambda_E read+write, ca
+    if kernel == "adj_e_w":
+        # gather_e, then the H-side weights: W_E read, lambda_H read+write, hcurl, W_H written
+        return cells * 76 + 8 * ph
     if kernel == "material_gradient":
         acc = 8 if args.accumulator == "float64" else 4
         return cells * (2 * comps * hist_b + 12) + ext * 2 * 2 * acc

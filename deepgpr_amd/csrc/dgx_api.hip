@@ -85,7 +85,7 @@ long long g_timing_count[kKCount];
 const char* const kKernelNames[kKCount] = {
     "coefficients", "fwd_h", "fwd_e", "sample", "quant_int8", "save_frame", "adj_receivers",
     "adj_source", "material_gradient", "adj_pml_e", "adj_e", "adj_pml_h", "adj_h", "tm_step_fwd",
-    "tm_step_adj", "tm_grad_reduce", "tm_src_reduce", "adj_weights_e", "adj_weights_h"};
+    "tm_step_adj", "tm_grad_reduce", "tm_src_reduce", "adj_weights_e", "adj_weights_h", "adj_e_w"};
 
 struct LaunchTimer {
     bool on;
@@ -420,6 +420,17 @@ bool tm2d_fused()
 {
     static const bool fused = [] {
         const char* v = std::getenv("DEEPGPR_AMD_TM2D_KERNELS");
+        return !(v && std::strcmp(v, "split") == 0);
+    }();
+    return fused;
+}
+
+// The 3D adjoint runs the E-side gather fused with the H-side weights;
+// DEEPGPR_AMD_3D_ADJOINT=split keeps the separate weight kernel (same bits).
+bool adj3d_fused()
+{
+    static const bool fused = [] {
+        const char* v = std::getenv("DEEPGPR_AMD_3D_ADJOINT");
         return !(v && std::strcmp(v, "split") == 0);
     }();
     return fused;
@@ -1054,6 +1065,8 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
         rg.ngroups = (int)gshot.size();
     }
 
+    const bool fuse3 = !tm && adj3d_fused();
+
     // fused 2D TM kernels: the groups again, listed for every (tile, shot)
     // whose staged region (tile plus halo) holds the cell
     DevBuf d_xoff, d_xgrp;
@@ -1124,10 +1137,11 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             p += across * g.pml[f];
         }
     } else if (!tm) {
-        // six channel-weight arrays shared by the E and H halves of the adjoint
-        if (!d_w.alloc((size_t)6 * g.nshot * g.cells * sizeof(float))) {
-            set_error("backward: out of device memory for the adjoint channel weights (%lld bytes)",
-                      (long long)6 * g.nshot * g.cells * 4);
+        // six channel-weight arrays: shared by the E and H halves of the
+        // adjoint, or one set per half when the gathers write the next weights
+        const long long wlen = (fuse3 ? 12LL : 6LL) * g.nshot * g.cells;
+        if (!d_w.alloc((size_t)wlen * sizeof(float))) {
+            set_error("backward: out of device memory for the adjoint channel weights (%lld bytes)", wlen * 4);
             return -1;
         }
     }
@@ -1226,6 +1240,12 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             TIMED(tmr, kKAdjE, launch_tm_adj_e(pl.order, tms.g, tms.f, tw, stream));
             if (a.has_cpml) TIMED(tmr, kKAdjPmlH, launch_tm_adj_pml(false, tms.g, tms.f, tw, stream));
             TIMED(tmr, kKAdjH, launch_tm_adj_h(pl.order, tms.g, tms.f, tw, stream));
+        } else if (fuse3) {
+            float* We = (float*)d_w.p;
+            float* Wh = We + 6LL * g.nshot * g.cells;
+            TIMED(tmr, kKAdjWeightsE, launch_adj_weights(pl.order, true, g, a, We, stream));
+            TIMED(tmr, kKAdjEW, launch_adj_gather_e_w(pl.order, g, a, We, Wh, stream));
+            TIMED(tmr, kKAdjH, launch_adj_gather_h(pl.order, g, a, Wh, stream));
         } else {
             float* W = (float*)d_w.p;
             TIMED(tmr, kKAdjWeightsE, launch_adj_weights(pl.order, true, g, a, W, stream));
@@ -1807,6 +1827,10 @@ DGX_API long long dgx_scratch_bytes(const dgx_solve* solve, int backward)
             for (int f = 0; f < 6; ++f)
                 for (int w = 0; w < 2; ++w) bytes += 4 * (psi_elems(g, true, f, w) + psi_elems(g, false, f, w));
         if (solve->accumulate_fp64) bytes += 16LL * (g.N[0] - 1) * (g.N[1] - 1) * (g.N[2] - 1);
+        // adjoint channel weights of the general (non-2D-TM) path
+        const bool tm_shape = (solve->storage_type & kFlagTm2d) && g.N[2] == 2 && solve->source_component == 2 &&
+                              pl.kind != kInt8;
+        if (!tm_shape) bytes += (adj3d_fused() ? 48LL : 24LL) * g.nshot * g.cells;
     }
     return bytes;
 }

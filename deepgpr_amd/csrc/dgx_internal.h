@@ -51,6 +51,7 @@ struct ReceiverGroups {
     int ngroups;
 };
 
+
 struct MatGrad {
     const float* ca;
     const float* cb;
@@ -276,12 +277,14 @@ void launch_add_f64(const double* a, float* out, long long count, hipStream_t st
 void launch_adj_weights(int order, bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st);
 void launch_adj_gather_e(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st);
 void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st);
+// the E-side gather fused with the H-side weights: W_E -> lambda_H -> W_H
+void launch_adj_gather_e_w(int order, const Grid& g, const AdjArgs& a, const float* We, float* Wh, hipStream_t st);
 
 // Kernel ids of the optional per-launch timing (dgx_set_kernel_timing).
 enum KernelId : int {
     kKCoeff = 0, kKFwdH, kKFwdE, kKSample, kKQuant, kKSaveFrame, kKAdjRx, kKAdjSrc, kKMatGrad, kKAdjPmlE,
     kKAdjE, kKAdjPmlH, kKAdjH, kKTmStepFwd, kKTmStepAdj, kKTmGradReduce, kKTmSrcReduce, kKAdjWeightsE,
-    kKAdjWeightsH, kKCount
+    kKAdjWeightsH, kKAdjEW, kKCount
 };
 
 }  // namespace dgx

@@ -467,22 +467,15 @@ __global__ __launch_bounds__(256) void k_add_f64_to_f32(const double* __restrict
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int chan_of(int u, int a) { return 2 * u + (a > u ? a - 1 : a); }
 
+// Channel weights of cell (c, shot s) from its lambda values lt3 (curl
+// coefficient cu; upd is the CPML source coefficient array): one store per
+// channel, and the cell's lambda psi advanced.
 template <bool ELECTRIC>
-__global__ __launch_bounds__(256) void k_adj_weights(Grid g, const float* __restrict__ curl,
-                                                     const float* __restrict__ upd, CF3 lam, Cpml cp,
-                                                     float* __restrict__ W, int has_cpml, ZRuns zr)
+__device__ __forceinline__ void adj_weights_cell(const Grid& g, float cu, const float* __restrict__ upd,
+                                                 const float (&lt3)[3], const Cpml& cp, float* __restrict__ W,
+                                                 bool cpml, const int (&c)[3], long long m, long long n, int s)
 {
-    int j, k;
-    if (!run_cell(g, zr, j, k)) return;
-    const int p = j * g.N[2] + k;
-    const int i = blockIdx.y, s = blockIdx.z;
-    const int c[3] = {i, j, k};
-    const long long m = (long long)i * g.st0 + p;
-    const long long n = (long long)s * g.cells + m;
     const long long cstride = (long long)g.nshot * g.cells;
-    const bool cpml = has_cpml != 0 && in_any_face(g, ELECTRIC, c);
-    const float cu = curl[m];
-    const float lt3[3] = {lam.c[0][n], lam.c[1][n], lam.c[2][n]};
     if (!cpml && plain_cell(g, 1, c)) {
         // every component updated and no CPML term: the curl terms alone
 #pragma unroll
@@ -543,6 +536,23 @@ __global__ __launch_bounds__(256) void k_adj_weights(Grid g, const float* __rest
     }
 }
 
+template <bool ELECTRIC>
+__global__ __launch_bounds__(256) void k_adj_weights(Grid g, const float* __restrict__ curl,
+                                                     const float* __restrict__ upd, CF3 lam, Cpml cp,
+                                                     float* __restrict__ W, int has_cpml, ZRuns zr)
+{
+    int j, k;
+    if (!run_cell(g, zr, j, k)) return;
+    const int p = j * g.N[2] + k;
+    const int i = blockIdx.y, s = blockIdx.z;
+    const int c[3] = {i, j, k};
+    const long long m = (long long)i * g.st0 + p;
+    const long long n = (long long)s * g.cells + m;
+    const bool cpml = has_cpml != 0 && in_any_face(g, ELECTRIC, c);
+    const float lt3[3] = {lam.c[0][n], lam.c[1][n], lam.c[2][n]};
+    adj_weights_cell<ELECTRIC>(g, curl[m], upd, lt3, cp, W, cpml, c, m, n, s);
+}
+
 // lambda_H += D-^T W_E (the E-side weights); lambda_E is scaled later.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_adj_gather_e(Grid g, const float* __restrict__ W, F3 lamH, ZRuns zr)
@@ -597,6 +607,43 @@ __global__ __launch_bounds__(256) void k_adj_gather_h(Grid g, const float* __res
         }
         lamE.c[t][n] = acc;
     }
+}
+
+// The E-side gather followed by the H-side weights of the same step, so that
+// the new lambda_H is used while it is in registers: lambda_H += D-^T W_E,
+// then W_H of the new lambda_H. W_E and W_H are separate arrays (every cell
+// reads W_E at its neighbours and writes only its own W_H channels).
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_adj_gather_e_w(Grid g, const float* __restrict__ We, F3 lamH, ZRuns zr,
+                                                        const float* __restrict__ hcurl,
+                                                        const float* __restrict__ hrhs, Cpml cph,
+                                                        float* __restrict__ Wh, int has_cpml)
+{
+    int j, k;
+    if (!run_cell(g, zr, j, k)) return;
+    const int p = j * g.N[2] + k;
+    const int i = blockIdx.y, s = blockIdx.z;
+    const int c[3] = {i, j, k};
+    const long long m = (long long)i * g.st0 + p;
+    const long long n = (long long)s * g.cells + m;
+    const long long st[3] = {g.st0, (long long)g.N[2], 1};
+    const long long cstride = (long long)g.nshot * g.cells;
+    float lt3[3];
+#pragma unroll
+    for (int u = 2; u >= 0; --u) {
+        float acc = lamH.c[u][n];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a == u) continue;
+            const float* Wc = We + chan_of(u, a) * cstride + n;
+            auto wf = [&](int off) { return Wc[off * st[a]]; };
+            acc += gather_bwd_T<ORDER>(wf, c[a], g.N[a]);
+        }
+        lamH.c[u][n] = acc;
+        lt3[u] = acc;
+    }
+    const bool cpml = has_cpml != 0 && in_any_face(g, false, c);
+    adj_weights_cell<false>(g, hcurl[m], hrhs, lt3, cph, Wh, cpml, c, m, n, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -789,6 +836,24 @@ void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float
         else if (order == 4)
             hipLaunchKernelGGL(k_adj_gather_h<4>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.run[q]);
         else hipLaunchKernelGGL(k_adj_gather_h<2>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.run[q]);
+    }
+}
+
+void launch_adj_gather_e_w(int order, const Grid& g, const AdjArgs& a, const float* We, float* Wh, hipStream_t st)
+{
+    const CellSweep cs = cell_sweep(g, order);
+    for (int q = 0; q < cs.count; ++q) {
+        const dim3 grid = cs.grid[q];
+        const ZRuns zr = cs.run[q];
+        if (order == 8)
+            hipLaunchKernelGGL(k_adj_gather_e_w<8>, grid, dim3(256), 0, st, g, We, a.lH, zr, a.hcurl, a.hrhs, a.cph,
+                               Wh, a.has_cpml);
+        else if (order == 4)
+            hipLaunchKernelGGL(k_adj_gather_e_w<4>, grid, dim3(256), 0, st, g, We, a.lH, zr, a.hcurl, a.hrhs, a.cph,
+                               Wh, a.has_cpml);
+        else
+            hipLaunchKernelGGL(k_adj_gather_e_w<2>, grid, dim3(256), 0, st, g, We, a.lH, zr, a.hcurl, a.hrhs, a.cph,
+                               Wh, a.has_cpml);
     }
 }
 

@@ -38,6 +38,9 @@ def estimate_compute_memory(nx: int, ny: int, nz: int, nt: int, nstep: int, pml,
         out["adjoint_fields"] = 6 * field
         out["adjoint_cpml"] = 2 * out["cpml"]
         out["gradients"] = 2 * cells * 4 + (2 * cells * 8 if fp64_accumulators else 0)
+        if nz > 1:
+            # 3D adjoint: six channel weights per cell for each half step
+            out["adjoint_weights"] = 12 * nstep * (nx + 1) * (ny + 1) * (nz + 1) * 4
     out["total"] = sum(out.values())
     return out
 

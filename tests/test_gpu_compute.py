@@ -99,7 +99,6 @@ def test_compute_c1_matches_oracle(oracle_built, cuda_device, monkeypatch, acc):
     np.testing.assert_allclose(eps.grad.cpu().numpy(), folded[:, :, 0].numpy(), rtol=2e-6, atol=1e-6)
 
 
-# the larger grids have cells whose adjoint weights the gathers form themselves
 # (11, 16, 90) and (12, 14, 96): rows long enough for the z-interior to run as its own launch
 @pytest.mark.parametrize("shape,order", [((14, 12, 10), 4), ((12, 56, 44), 4), ((20, 40, 36), 8), ((16, 30, 26), 2),
                                          ((11, 16, 90), 4), ((12, 14, 96), 8)])
