@@ -122,14 +122,55 @@ def _tm2d_applies(device, spatial_mode: int, source_direction: int, states, user
     return int(sum(t.detach().count_nonzero() for t in inactive if t.numel())) == 0
 
 
-def save_forward_wavefield(e_saved: torch.Tensor, directory: str, when: datetime, metadata=None) -> str:
-    """Write ``forward_wavefield_HH-MM.pt``: the bare history or, with metadata,
-    ``{"E_saved": ..., **metadata}`` (src/DeepGPR/postprocessing/io.py:60-92)."""
-    os.makedirs(directory, exist_ok=True)
-    path = os.path.join(directory, f"forward_wavefield_{when.strftime('%H-%M')}.pt")
-    payload = e_saved.detach().cpu() if metadata is None else {"E_saved": e_saved.detach().cpu(), **metadata}
-    torch.save(payload, path)
+def wavefield_directory(value) -> Optional[str]:
+    """``save_forward_wavefield_path`` as a directory path, or None. A value that
+    is not path-like raises TypeError, an existing non-directory
+    NotADirectoryError (src/DeepGPR/postprocessing/io.py:17-37)."""
+    if value is None:
+        return None
+    try:
+        path = os.path.expanduser(os.fspath(value))
+    except TypeError as exc:
+        raise TypeError("save_forward_wavefield_path must be a string, path-like object, or None.") from exc
+    if os.path.exists(path) and not os.path.isdir(path):
+        raise NotADirectoryError(f"save_forward_wavefield_path is not a directory: {path}")
     return path
+
+
+def _claim_wavefield_file(directory: str, label: str) -> str:
+    """Create ``forward_wavefield_<label>.pt``, or the first free
+    ``forward_wavefield_<label>_NN.pt`` (NN = 01, 02, ...), with exclusive
+    creation so that concurrent savers never share a file
+    (src/DeepGPR/postprocessing/io.py:40-57)."""
+    n = 0
+    while True:
+        name = f"forward_wavefield_{label}.pt" if n == 0 else f"forward_wavefield_{label}_{n:02d}.pt"
+        path = os.path.join(directory, name)
+        try:
+            fd = os.open(path, os.O_CREAT | os.O_EXCL | os.O_WRONLY, 0o666)
+        except FileExistsError:
+            n += 1
+            continue
+        os.close(fd)
+        return path
+
+
+def save_forward_wavefield(e_saved: torch.Tensor, directory: str, when: datetime, metadata=None) -> str:
+    """Write a CPU copy of the history to ``forward_wavefield_HH-MM[_NN].pt`` and
+    return its absolute path: the bare tensor or, with metadata,
+    ``{"wavefield": tensor, **metadata}``. A failed write removes the file
+    (src/DeepGPR/postprocessing/io.py:60-92)."""
+    os.makedirs(directory, exist_ok=True)
+    path = _claim_wavefield_file(directory, when.strftime("%H-%M"))
+    data = e_saved.detach().to(device="cpu")
+    payload = data if metadata is None else {"wavefield": data, **metadata}
+    try:
+        torch.save(payload, path)
+    except BaseException:
+        if os.path.exists(path):
+            os.remove(path)
+        raise
+    return os.path.realpath(path)
 
 
 @dataclass
@@ -188,7 +229,7 @@ def prepare_compute(device, dx=None, dt=None, source_amplitudes=None, source_loc
     for name, flag in (("print_parameters", print_parameters), ("save_wavefield_history", save_wavefield_history)):
         if not isinstance(flag, bool):
             raise TypeError(f"{name} must be a bool.")
-    out_dir = None if save_forward_wavefield_path is None else os.fspath(save_forward_wavefield_path)
+    out_dir = wavefield_directory(save_forward_wavefield_path)
     when = datetime.now() if out_dir is not None else None
     automatic = (isinstance(model_gradient_sampling_interval, str)
                  and model_gradient_sampling_interval.lower() == AUTO_SAMPLING_INTERVAL)
