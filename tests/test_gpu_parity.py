@@ -143,6 +143,33 @@ def _tm2d_problem(h, NX, NY, order, seed, nsrc=2, nrx=4):
 
 
 @pytest.mark.parametrize("order", [2, 4, 8])
+def test_tm2d_large_grid_matches_oracle(oracle_built, cuda_device, order):
+    """A grid with tiles whose staged region (tile and halo) avoids the CPML
+    and the grid faces, with sources and receivers inside such tiles and on
+    tile edges."""
+    h = oracle_built
+    print(f"[tm2d large] order={order}", flush=True)
+    p = _tm2d_problem(h, 90, 210, order, seed=60 + order, nsrc=3, nrx=6)
+    for s in range(2):
+        p.src_loc[s, 0] = [20 + 5 * s, 70, 0]
+        p.src_loc[s, 1] = [47, 127 + s, 0]
+        p.rx_loc[s, 0] = [33, 100, 0]
+        p.rx_loc[s, 1] = [48, 128, 0]
+        p.rx_loc[s, 2] = [64, 64 + s, 0]
+    p.mode, p.S = 3, 1
+    p.storage_type = _storage(h, "e_only")
+    cpu = h.run_cpu(p, "oracle")
+    p.storage_type |= TM2D
+    gpu = h.run_gpu(p, api="dgx")
+    bad = h.bitwise_diff(gpu, cpu, h.KEYS_FWD + ("g_src", "g_eps", "g_sig"))
+    assert not bad, bad
+    active = [2, 3, 4] + [6 + k for k in range(24) if k not in INACTIVE_PHI]
+    for k in active:
+        assert np.array_equal(gpu["L"][k], cpu["L"][k]), k
+    assert np.array_equal(gpu["E_saved"], cpu["E_saved"])
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
 @pytest.mark.parametrize("mode,S,st", [(2, 1, "e_only"), (3, 1, "e_only"), (3, 2, "f32"), (3, 1, "bf16")])
 def test_tm2d_fast_path_matches_general_oracle(oracle_built, cuda_device, order, mode, S, st):
     h = oracle_built
