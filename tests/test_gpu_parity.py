@@ -1,8 +1,8 @@
 """HIP kernels vs the CPU oracle (float32 restatement of the reference CPU
 backend, itself bitwise-checked against the reference build). With
 -ffp-contract=off and the reference operation order the HIP path is expected
-to match bit for bit; every case asserts that, and separately the
-north-star tolerances (receiver traces rel-L2 <= 1e-5, gradients <= 1e-4)."""
+to match bit for bit; every case asserts that, and separately the stated
+floating-point tolerances (receiver traces rel-L2 <= 1e-5, gradients <= 1e-4)."""
 import numpy as np
 import pytest
 
