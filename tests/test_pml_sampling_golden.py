@@ -1,0 +1,40 @@
+"""build_pml_coeffs and the "auto" sampling interval against vectors written
+by the reference's own functions (tests/golden/make_pml_sampling_golden.py:
+src/DeepGPR/solver/pml.py:227-298, src/DeepGPR/solver/sampling.py:58-134).
+The coefficients must match bit for bit in float32 and float64."""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+from make_pml_sampling_golden import OUT, PML_CASES, SAMPLING_CASES  # noqa: E402
+
+from deepgpr_amd.solver.pml import build_pml_coeffs  # noqa: E402
+from deepgpr_amd.solver.sampling import recommended_sampling_interval, source_max_frequency  # noqa: E402
+
+GOLD = np.load(OUT, allow_pickle=False)
+
+
+@pytest.mark.parametrize("k", range(len(PML_CASES)))
+def test_build_pml_coeffs_bitwise(k):
+    nx, ny, nz, th, dx, dt, vmax, dt_name = PML_CASES[k]
+    kw = {} if vmax is None else {"max_velocity": vmax}
+    res = build_pml_coeffs(None, None, dt, dx, nx, ny, nz, torch.tensor(th), torch.device("cpu"),
+                           getattr(torch, dt_name), **kw)
+    assert len(res) == 18
+    for i, t in enumerate(res):
+        ref = GOLD[f"pml{k}_{i}"]
+        got = t.detach().cpu().numpy()
+        assert got.dtype == ref.dtype and got.shape == ref.shape, (i, got.dtype, ref.dtype, got.shape, ref.shape)
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), (k, i)
+
+
+@pytest.mark.parametrize("k", range(len(SAMPLING_CASES)))
+def test_auto_sampling_interval(k):
+    w = torch.from_numpy(GOLD[f"wave{k}"])
+    dt = SAMPLING_CASES[k][2]
+    assert source_max_frequency(w, dt) == float(GOLD[f"fmax{k}"])
+    assert recommended_sampling_interval(w, dt) == int(GOLD[f"interval{k}"])
