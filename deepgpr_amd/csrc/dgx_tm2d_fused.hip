@@ -135,6 +135,13 @@ __device__ __forceinline__ ACC sum_shots(const float* __restrict__ v, long long 
     return acc;
 }
 
+// Cell h of the history box: its output index in the (NX-1, NY-1) gradient.
+__device__ __forceinline__ long long terms_out(const Tm2dGrid& g, const Box& b, long long h)
+{
+    const int i = (int)(h / b.ny) + b.x0, j = (int)(h % b.ny) + b.y0;
+    return (long long)i * (g.NY - 1) + j;
+}
+
 // Sum of the per-shot material terms of cell h of the history box, in shot
 // order, added to the accumulators (one saved step).
 template <typename ACC>
@@ -142,8 +149,7 @@ __device__ __forceinline__ void reduce_terms(const Tm2dGrid& g, const Box& b, co
                                              long long h, int want_eps, int want_sig, ACC* __restrict__ geps,
                                              ACC* __restrict__ gsig)
 {
-    const int i = (int)(h / b.ny) + b.x0, j = (int)(h % b.ny) + b.y0;
-    const long long out = (long long)i * (g.NY - 1) + j;
+    const long long out = terms_out(g, b, h);
     const long long sstride = (long long)g.nshot * b.cells;
     if (want_eps) geps[out] += sum_shots<ACC>(terms + h, b.cells, g.nshot);
     if (want_sig) gsig[out] += sum_shots<ACC>(terms + sstride + h, b.cells, g.nshot);
