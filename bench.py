@@ -171,7 +171,7 @@ def kernel_bytes(kernel: str, w, args) -> float:
 # accumulator, shots per call): traffic depends on all of them. The commands
 # and per-kernel tables are in the profiles/*.md file named with each entry.
 MEASURED_TRAFFIC = {
-    ("c2", "tm_step_adj", "float32", "float64", 20): (121.3e6, "profiles/c2_fwi_2d_o4_tm_step_kernels.md"),
+    ("c2", "tm_step_adj", "float32", "float64", 20): (121.9e6, "profiles/c2_fwi_2d_o4_tm_step_kernels.md"),
     ("c2", "tm_step_fwd", "float32", "float64", 20): (90.4e6, "profiles/c2_fwi_2d_o4_tm_step_kernels.md"),
     ("c3", "fwd_e", "float32", "float64", 1): (1800.9e6, "profiles/c3_layered_3d_o4_kernels.md"),
     ("c3", "fwd_h", "float32", "float64", 1): (1510.2e6, "profiles/c3_layered_3d_o4_kernels.md"),
