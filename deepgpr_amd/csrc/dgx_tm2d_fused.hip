@@ -86,6 +86,20 @@ __device__ __forceinline__ bool hy_up(const Tm2dGrid& g, int i, int j)
     return (g.NY - 1) != 1 && i < g.NX - 1 && j > 0 && j < g.NY - 1;
 }
 
+// Element idx of an array within 4 GiB of its base: the byte offset is a
+// 32-bit value, so a wave-uniform base is addressed as SGPR base + 32-bit
+// VGPR offset instead of a 64-bit address per lane.
+template <typename T>
+__device__ __forceinline__ T& at32(T* base, unsigned idx)
+{
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)(idx * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ const T& at32(const T* base, unsigned idx)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(idx * (unsigned)sizeof(T)));
+}
+
 struct Cpml4 {
     float ra1, rb, re, rf;
 };
