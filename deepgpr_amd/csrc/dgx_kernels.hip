@@ -353,6 +353,50 @@ __global__ __launch_bounds__(256) void k_quant_int8(Grid g, CF3 E, const float* 
     }
 }
 
+// The same for 64-cell tiles (the default 4x4x4 and 8x8x1 blocks), one
+// wavefront per tile and four tiles per workgroup. The maximum runs through
+// wavefront shuffles in the pairing of the tree above (lane i takes lane
+// i + stride when that value is larger, stride 32 down to 1), so the scale is
+// the same value, NaN inputs included, with no barrier.
+__global__ __launch_bounds__(256) void k_quant_int8_w64(Grid g, CF3 E, const float* __restrict__ src,
+                                                        long long src_cstride, Int8Write w)
+{
+    const int local = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + (int)(threadIdx.x >> 6), s = blockIdx.y, ci = blockIdx.z;
+    if (tile >= w.tiles) return;  // whole wavefronts
+    const int comp = w.mode == 3 ? ci : 2;
+    const int tz = tile % w.nbz, ty = (tile / w.nbz) % w.nby, tx = tile / (w.nbz * w.nby);
+    const int lx = local / (w.by * w.bz), ly = (local / w.bz) % w.by, lz = local % w.bz;
+    const int x = tx * w.bx + lx, y = ty * w.by + ly, z = tz * w.bz + lz;
+    const bool valid = x < w.box.nx && y < w.box.ny && z < w.box.nz;
+    const long long h = valid ? ((long long)x * w.box.ny + y) * w.box.nz + z : 0;
+    float v = 0.0f;
+    if (valid) {
+        if (src) {
+            v = src[ci * src_cstride + (long long)s * w.box.cells + h];
+        } else {
+            long long n = (long long)s * g.cells + (long long)(x + w.box.x0) * g.st0 +
+                          (long long)(y + w.box.y0) * g.N[2] + (z + w.box.z0);
+            v = E.c[comp][n];
+        }
+    }
+    float m = v < 0.0f ? -v : v;
+#pragma unroll
+    for (int stride = 32; stride > 0; stride >>= 1) {
+        const float o = __shfl_down(m, stride, 64);
+        if (local < stride && o > m) m = o;
+    }
+    const float peak = __shfl(m, 0, 64);
+    const float scale = peak >= 1.0e-30f ? peak / 127.0f : 1.0f;
+    const long long snap = (long long)g.nshot * w.box.cells;
+    if (local == 0) w.scale[((long long)ci * w.frames + w.frame) * g.nshot * w.tiles + (long long)s * w.tiles + tile] = scale;
+    if (valid) {
+        float r = rintf(v / scale);
+        r = fminf(fmaxf(r, -127.0f), 127.0f);
+        w.q[((long long)ci * w.frames + w.frame) * snap + (long long)s * w.box.cells + h] = (signed char)(int)r;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint: receivers^T and source^T (deepgpr_cpu.c:1479-1534)
 // ---------------------------------------------------------------------------
@@ -766,6 +810,11 @@ void launch_quant_int8(const Grid& g, CF3 E, const float* src, long long src_cst
                        hipStream_t st)
 {
     int comps = w.mode == 3 ? 3 : 1;
+    if (w.bx * w.by * w.bz == 64) {
+        dim3 grid((unsigned)((w.tiles + 3) / 4), (unsigned)g.nshot, (unsigned)comps);
+        hipLaunchKernelGGL(k_quant_int8_w64, grid, dim3(256), 0, st, g, E, src, src_cstride, w);
+        return;
+    }
     dim3 grid((unsigned)w.tiles, (unsigned)g.nshot, (unsigned)comps);
     hipLaunchKernelGGL(k_quant_int8, grid, dim3(w.bx * w.by * w.bz), 0, st, g, E, src, src_cstride, w);
 }

@@ -69,7 +69,8 @@ def test_gpu_adjoint_receivers_match_oracle(oracle_built, cuda_device, order, rx
 
 def test_gpu_int8_history_matches_oracle(oracle_built, cuda_device):
     h = oracle_built
-    for NX, NY, NZ, block in ((14, 13, 12, (4, 4, 4)), (20, 18, 2, (8, 8, 1))):
+    # 64-cell tiles (one wavefront per tile) and a 16-cell tile (the general kernel)
+    for NX, NY, NZ, block in ((14, 13, 12, (4, 4, 4)), (20, 18, 2, (8, 8, 1)), (14, 13, 12, (2, 4, 2))):
         p = h.make_problem(NX, NY, NZ, order=4, nt=9, nshot=2, nsrc=1, nrx=3, seed=3)
         p.storage_type = h.int8_storage_type(block)
         for e_only in (False, True):
