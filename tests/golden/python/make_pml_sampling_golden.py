@@ -1,4 +1,4 @@
-"""Writes tests/golden/pml_sampling_golden.npz: input/output vectors of two
+"""Writes tests/golden/python/pml_sampling_golden.npz: input/output vectors of two
 Python functions of the reference that no reference test stores values for,
 
 - build_pml_coeffs (src/DeepGPR/solver/pml.py:227-298): the float32/float64
@@ -8,7 +8,7 @@ Python functions of the reference that no reference test stores values for,
   (src/DeepGPR/solver/sampling.py:58-134): the "auto" sampling interval.
 
 Run once, in a container holding the reference checkout, with
-``python tests/golden/make_pml_sampling_golden.py [reference_src_dir]``; the
+``python tests/golden/python/make_pml_sampling_golden.py [reference_src_dir]``; the
 test suite only reads the .npz. The reference modules are loaded one by one
 under empty parent packages, so no package ``__init__`` runs and no native
 library of the reference is loaded.

@@ -1,5 +1,5 @@
 """build_pml_coeffs and the "auto" sampling interval against vectors written
-by the reference's own functions (tests/golden/make_pml_sampling_golden.py:
+by the reference's own functions (tests/golden/python/make_pml_sampling_golden.py:
 src/DeepGPR/solver/pml.py:227-298, src/DeepGPR/solver/sampling.py:58-134).
 The coefficients must match bit for bit in float32 and float64."""
 import os
@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import torch
 
-sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden", "python"))
 from make_pml_sampling_golden import OUT, PML_CASES, SAMPLING_CASES  # noqa: E402
 
 from deepgpr_amd.solver.pml import build_pml_coeffs  # noqa: E402
