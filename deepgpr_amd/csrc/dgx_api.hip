@@ -414,6 +414,57 @@ bool tm2d_applies(const dgx_solve* P, const Plan& pl, const std::vector<int>& sl
     return true;
 }
 
+// Forward-only fused 2D TM solves replay their step loop as a graph of
+// kGraphSteps step launches (DEEPGPR_AMD_TM2D_GRAPHS=0 turns this off).
+constexpr int kGraphSteps = 32;  // even: a replay leaves the ping-pong parity unchanged
+
+bool tm2d_graphs()
+{
+    static const bool on = [] {
+        const char* v = std::getenv("DEEPGPR_AMD_TM2D_GRAPHS");
+        return !(v && std::strcmp(v, "0") == 0);
+    }();
+    return on;
+}
+
+// Graphs are captured on a stream of the calling thread (capture is not
+// possible on the null stream) and launched on the caller's stream.
+hipStream_t capture_stream()
+{
+    thread_local hipStream_t streams[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
+        streams[dev] = nullptr;
+    return streams[dev];
+}
+
+// An executable graph is destroyed once the event recorded after its last
+// launch has completed.
+struct RetiredGraph {
+    hipGraphExec_t exec;
+    hipGraph_t graph;
+    hipEvent_t done;
+};
+std::mutex g_graph_mu;
+std::vector<RetiredGraph> g_graphs;
+
+void sweep_graphs()
+{
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    for (size_t q = 0; q < g_graphs.size();) {
+        if (hipEventQuery(g_graphs[q].done) != hipSuccess) {
+            ++q;
+            continue;
+        }
+        (void)hipGraphExecDestroy(g_graphs[q].exec);
+        (void)hipGraphDestroy(g_graphs[q].graph);
+        (void)hipEventDestroy(g_graphs[q].done);
+        g_graphs.erase(g_graphs.begin() + (long)q);
+    }
+    (void)hipGetLastError();  // a pending event's "not ready" is no error of the caller's
+}
+
 // The split per-operator TM kernels stay selectable for comparison
 // (DEEPGPR_AMD_TM2D_KERNELS=split); the fused step kernels are the default.
 bool tm2d_fused()
@@ -859,7 +910,59 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         return -1;
     }
 
-    for (int it = 0; it < nt; ++it) {
+    // Forward-only fused solves: kGraphSteps steps captured once (each node
+    // reads its step from d_step), replayed nt / kGraphSteps times; the
+    // remaining steps are launched one by one below.
+    int it0 = 0;
+    DevBuf d_step;
+    const bool graphed = fused && !pl.history && !tmr.on && tm2d_graphs() && nt >= 2 * kGraphSteps &&
+                         (P->nreceiver <= 0 || fused_rx);
+    if (graphed) {
+        hipStream_t cs = capture_stream();
+        if (cs && d_step.alloc(sizeof(int))) {
+            int* step = (int*)d_step.p;
+            HistWrite none{};
+            none.mode = pl.comps == 3 ? 3 : 2;
+            none.box = pl.box;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                for (int k = 0; k < kGraphSteps; ++k) {
+                    launch_tm_step_fwd(pl.order, tms.g, tms.f, tms.pp, a.src, trx, none, k, cs, step);
+                    tms.flip();
+                }
+                launch_step_counter(step, kGraphSteps, true, cs);
+                ok = hipStreamEndCapture(cs, &graph) == hipSuccess && graph;
+            }
+            ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+            if (ok) {
+                launch_step_counter(step, 0, false, stream);
+                const int reps = nt / kGraphSteps;
+                for (int r = 0; r < reps && ok; ++r) ok = hipGraphLaunch(exec, stream) == hipSuccess;
+                hipEvent_t done = nullptr;
+                if (ok && hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess &&
+                    hipEventRecord(done, stream) == hipSuccess) {
+                    std::lock_guard<std::mutex> lock(g_graph_mu);
+                    g_graphs.push_back({exec, graph, done});
+                    exec = nullptr;
+                    graph = nullptr;
+                }
+                if (!ok) {
+                    set_error("forward: graph launch failed");
+                    return -1;
+                }
+                it0 = reps * kGraphSteps;
+            }
+            if (exec || graph) {  // not launched, or its completion event could not be recorded
+                if (it0) (void)hipStreamSynchronize(stream);
+                if (exec) (void)hipGraphExecDestroy(exec);
+                if (graph) (void)hipGraphDestroy(graph);
+            }
+            (void)hipGetLastError();  // a failed capture falls back to launches one by one
+        }
+    }
+    for (int it = it0; it < nt; ++it) {
         const bool save = pl.history && (it % S) == 0;
         const int ts = it / S;
         HistWrite hw{};
@@ -941,6 +1044,7 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         DGX_TRY(hipStreamSynchronize(stream), "forward: stream synchronize");
         tmr.collect();
     }
+    sweep_graphs();
     return 0;
 }
 

@@ -227,7 +227,9 @@ void launch_clear_spans(const ClearSpans& sp, hipStream_t st);
 
 void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
                         const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
-                        hipStream_t st);
+                        hipStream_t st, const int* it_base = nullptr);
+// *it = value, or *it += value (one thread): the step counter of a replayed graph
+void launch_step_counter(int* it, int value, bool add, hipStream_t st);
 void launch_tm_step_adj(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
                         const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it, bool fp64,
                         hipStream_t st);

@@ -118,13 +118,17 @@ __device__ __forceinline__ Cpml4 cpml_at(const float* R, int p, int q)
 // (scalar); per-lane column data (the halo column a lane also covers, y-face
 // indices) is computed once.
 // ---------------------------------------------------------------------------
+// The time step is it_off, plus *it_base when it_base is set (a replayed
+// graph of several steps reads the step of its first node from the device).
 template <int ORDER>
 __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
-                                                     TmReceiversFwd rx, HistWrite hist, int it)
+                                                     TmReceiversFwd rx, HistWrite hist, int it_off,
+                                                     const int* __restrict__ it_base)
 {
     const int tile = step_tile(g.ftiles);
     if (tile < 0) return;
     const int s = blockIdx.y;
+    const int it = it_off + (it_base ? *it_base : 0);
 #include "dgx_tm2d_tile_fwd.inc"
 }
 
@@ -416,12 +420,24 @@ static dim3 step_grid(const Tm2dGrid& g) { return dim3((unsigned)((g.ftiles + 7)
 
 void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
                         const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
-                        hipStream_t st)
+                        hipStream_t st, const int* it_base)
 {
     const dim3 grid = step_grid(g);
-    if (order == 8) hipLaunchKernelGGL(k_tm_step_fwd<8>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it);
-    else if (order == 4) hipLaunchKernelGGL(k_tm_step_fwd<4>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it);
-    else hipLaunchKernelGGL(k_tm_step_fwd<2>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it);
+    if (order == 8)
+        hipLaunchKernelGGL(k_tm_step_fwd<8>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
+    else if (order == 4)
+        hipLaunchKernelGGL(k_tm_step_fwd<4>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
+    else hipLaunchKernelGGL(k_tm_step_fwd<2>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
+}
+
+__global__ void k_step_counter(int* __restrict__ it, int value, int add)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *it = add ? *it + value : value;
+}
+
+void launch_step_counter(int* it, int value, bool add, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step_counter, dim3(1), dim3(64), 0, st, it, value, add ? 1 : 0);
 }
 
 template <typename ACC>
