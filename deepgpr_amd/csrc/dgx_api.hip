@@ -918,6 +918,9 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     const bool graphed = fused && !pl.history && !tmr.on && tm2d_graphs() && nt >= 2 * kGraphSteps &&
                          (P->nreceiver <= 0 || fused_rx);
     if (graphed) {
+        // earlier launch errors are reported here: the fallback below clears
+        // whatever a failed capture leaves behind
+        DGX_TRY(hipGetLastError(), "forward: kernel launch");
         hipStream_t cs = capture_stream();
         if (cs && d_step.alloc(sizeof(int))) {
             int* step = (int*)d_step.p;
@@ -941,12 +944,15 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
                 const int reps = nt / kGraphSteps;
                 for (int r = 0; r < reps && ok; ++r) ok = hipGraphLaunch(exec, stream) == hipSuccess;
                 hipEvent_t done = nullptr;
-                if (ok && hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess &&
-                    hipEventRecord(done, stream) == hipSuccess) {
-                    std::lock_guard<std::mutex> lock(g_graph_mu);
-                    g_graphs.push_back({exec, graph, done});
-                    exec = nullptr;
-                    graph = nullptr;
+                if (ok && hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess) {
+                    if (hipEventRecord(done, stream) == hipSuccess) {
+                        std::lock_guard<std::mutex> lock(g_graph_mu);
+                        g_graphs.push_back({exec, graph, done});
+                        exec = nullptr;
+                        graph = nullptr;
+                    } else {
+                        (void)hipEventDestroy(done);
+                    }
                 }
                 if (!ok) {
                     set_error("forward: graph launch failed");
