@@ -487,7 +487,21 @@ bool adj3d_fused()
     return fused;
 }
 
-Tm2dGrid tm_grid_of(const Plan& pl)
+// Tile rows of a single fused TM solve: kTmBISmall when its kTmBI-row tiles
+// over all shots number fewer than kTmSmallTiles (too few workgroups to fill
+// the GPU, so the shorter per-workgroup chain of the smaller tile wins), else
+// kTmBI. DEEPGPR_AMD_TM2D_TILE_ROWS=8 or 16 forces one; it is read on every
+// call, so one process can run both.
+int tm2d_tile_rows(const Plan& pl)
+{
+    const char* v = std::getenv("DEEPGPR_AMD_TM2D_TILE_ROWS");
+    if (v && std::strcmp(v, "8") == 0) return kTmBISmall;
+    if (v && std::strcmp(v, "16") == 0) return kTmBI;
+    const long long tiles = (long long)((pl.g.N[0] + kTmBI - 1) / kTmBI) * ((pl.g.N[1] + kTmBJ - 1) / kTmBJ);
+    return tiles * pl.g.nshot < kTmSmallTiles ? kTmBISmall : kTmBI;
+}
+
+Tm2dGrid tm_grid_of(const Plan& pl, int bi)
 {
     Tm2dGrid g{};
     g.NX = pl.g.N[0];
@@ -495,7 +509,8 @@ Tm2dGrid tm_grid_of(const Plan& pl)
     g.plane = g.NX * g.NY;
     g.tiles = (g.plane + 255) / 256;
     g.ntj = (g.NY + kTmBJ - 1) / kTmBJ;
-    g.ftiles = ((g.NX + kTmBI - 1) / kTmBI) * g.ntj;
+    g.bi = bi;
+    g.ftiles = ((g.NX + bi - 1) / bi) * g.ntj;
     g.nshot = pl.g.nshot;
     for (int a = 0; a < 3; ++a) g.d[a] = pl.g.d[a];
     for (int f4 = 0; f4 < 4; ++f4) g.pml[f4] = pl.g.pml[f4];
@@ -503,8 +518,8 @@ Tm2dGrid tm_grid_of(const Plan& pl)
     return g;
 }
 
-int tm_tile_of(const Tm2dGrid& g, int i, int j) { return (i / kTmBI) * g.ntj + j / kTmBJ; }
-int tm_local(int i, int j) { return (i % kTmBI) * kTmBJ + j % kTmBJ; }
+int tm_tile_of(const Tm2dGrid& g, int i, int j) { return (i / g.bi) * g.ntj + j / kTmBJ; }
+int tm_local(const Tm2dGrid& g, int i, int j) { return (i % g.bi) * kTmBJ + j % kTmBJ; }
 
 // Compact buffers of the 2D TM path: 3 fields (nshot * plane; 6 with the
 // fused kernels' ping-pong copies) and 5 coefficient planes, plus the second
@@ -518,10 +533,10 @@ struct Tm2dState {
     float* home_h[4] = {};
     long long psi_len_e[4] = {}, psi_len_h[4] = {};
     DevBuf buf, alt;
-    int setup(const Plan& pl, const dgx_solve* P, hipStream_t stream, bool use_fused)
+    int setup(const Plan& pl, const dgx_solve* P, hipStream_t stream, bool use_fused, int bi)
     {
         fused = use_fused;
-        g = tm_grid_of(pl);
+        g = tm_grid_of(pl, bi);
         const long long field = (long long)g.nshot * g.plane;
         const int nf = fused ? 6 : 3;
         if (!buf.alloc(((size_t)nf * field + 5 * (size_t)g.plane) * sizeof(float))) return -1;
@@ -629,7 +644,7 @@ void tm_source_lists(const Tm2dGrid& g, const std::vector<int>& sloc, int nsrc, 
         for (int k = 0; k < nsrc; ++k) {
             const int* L = &sloc[((size_t)s * nsrc + k) * 3];
             tmp.push_back({(long long)tm_tile_of(g, L[0], L[1]) + (long long)g.ftiles * s,
-                           make_int2(tm_local(L[0], L[1]), k)});
+                           make_int2(tm_local(g, L[0], L[1]), k)});
         }
     std::stable_sort(tmp.begin(), tmp.end(),
                      [](const std::pair<long long, int2>& a, const std::pair<long long, int2>& b) {
@@ -652,7 +667,7 @@ void tm_receiver_lists(const Tm2dGrid& g, const std::vector<int>& rloc, int nrx,
         for (int r = 0; r < nrx; ++r) {
             const int* L = &rloc[((size_t)s * nrx + r) * 3];
             tmp.push_back({(long long)tm_tile_of(g, L[0], L[1]) + (long long)g.ftiles * s,
-                           make_int2(tm_local(L[0], L[1]), r)});
+                           make_int2(tm_local(g, L[0], L[1]), r)});
         }
     std::stable_sort(tmp.begin(), tmp.end(), [](const std::pair<long long, int2>& x,
                                                 const std::pair<long long, int2>& y) { return x.first < y.first; });
@@ -674,13 +689,14 @@ void tm_receiver_groups(const Tm2dGrid& g, int order, const std::vector<int>& go
 {
     const int he = tm_halo(order);
     const int lc = kTmBJ + 2 * he;
-    const int nti = (g.NX + kTmBI - 1) / kTmBI;
+    const int bi = g.bi;
+    const int nti = (g.NX + bi - 1) / bi;
     std::vector<std::pair<long long, int4>> tmp;
     for (size_t q = 0; q < gshot.size(); ++q) {
         const int i = (int)(gcell[q] / g.NY), j = (int)(gcell[q] % g.NY);
-        for (int ti = std::max(0, (i - he) / kTmBI - 1); ti <= std::min(nti - 1, (i + he) / kTmBI); ++ti) {
-            const int r = i - (ti * kTmBI - he);
-            if (r < 0 || r >= kTmBI + 2 * he) continue;
+        for (int ti = std::max(0, (i - he) / bi - 1); ti <= std::min(nti - 1, (i + he) / bi); ++ti) {
+            const int r = i - (ti * bi - he);
+            if (r < 0 || r >= bi + 2 * he) continue;
             for (int tj = std::max(0, (j - he) / kTmBJ - 1); tj <= std::min(g.ntj - 1, (j + he) / kTmBJ); ++tj) {
                 const int c = j - (tj * kTmBJ - he);
                 if (c < 0 || c >= lc) continue;
@@ -790,7 +806,7 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     const bool inject = P->nsource > 0 && P->source_component >= 0 && P->source_component <= 2;
     const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
     const bool fused = tm && tm2d_fused();
-    const Tm2dGrid tg = tm_grid_of(pl);
+    const Tm2dGrid tg = tm_grid_of(pl, tm2d_tile_rows(pl));
     const int tm_tiles = (g.N[0] * g.N[1] + 255) / 256;
     long long nblocks = tm ? (long long)tm_tiles * g.nshot : sweep.blocks;
     std::vector<int> boff((size_t)nblocks + 1, 0);
@@ -905,7 +921,7 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     a.src = src;
     const size_t eb = elem_bytes(pl.kind);
     Tm2dState tms;
-    if (tm && tms.setup(pl, P, stream, fused)) {
+    if (tm && tms.setup(pl, P, stream, fused, tg.bi)) {
         set_error("forward: out of device memory for the 2D TM state");
         return -1;
     }
@@ -1141,7 +1157,7 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
 
     const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
     const bool fused = tm && tm2d_fused();
-    const Tm2dGrid tg = tm_grid_of(pl);
+    const Tm2dGrid tg = tm_grid_of(pl, tm2d_tile_rows(pl));
     Grid gc = g;  // compact 2D TM addressing for the receiver/source kernels
     if (tm) {
         gc.st0 = g.N[1];
@@ -1303,7 +1319,7 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     mg.box = pl.box;
     const float geo = want_src ? g.d[P->source_component] / (g.d[0] * g.d[1] * g.d[2]) : 0.0f;
     Tm2dState tms;
-    if (tm && tms.setup(pl, P, stream, fused)) {
+    if (tm && tms.setup(pl, P, stream, fused, tg.bi)) {
         set_error("backward: out of device memory for the 2D TM state");
         return -1;
     }
@@ -1575,7 +1591,7 @@ int batch_add(Batch* b, const dgx_solve* P, const dgx_solve* G, hipStream_t stre
     I.order = pl.order;
     I.gradient = gradient;
     launch_coefficients(g, P->eps_r_pad, P->sigma_pad, P->mu_r_pad, P->update_coeff, P->dt, stream);
-    if (I.tms.setup(pl, P, stream, true)) {
+    if (I.tms.setup(pl, P, stream, true, kTmBI)) {  // the batch kernels use kTmBI-row tiles
         set_error("batch: out of device memory for an instance");
         return -1;
     }

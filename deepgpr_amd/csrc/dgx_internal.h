@@ -94,6 +94,7 @@ struct Tm2dGrid {
     int tiles;   // ceil(plane / 256)
     int ftiles;  // tiles of the fused step kernels: nti * ntj
     int ntj;     // kTmBJ-wide tile columns
+    int bi;      // tile rows of the fused step kernels: kTmBI, or kTmBISmall for small solves
     int nshot;
     float d[3];
     int pml[4];  // x0, xm, y0, ym
@@ -127,6 +128,11 @@ struct Tm2dWeights {
 #define DGX_TM_ADJ_THREADS 512
 #endif
 constexpr int kTmBI = DGX_TM_TILE_ROWS;
+// Single solves whose kTmBI-row tiles (all shots) number fewer than
+// kTmSmallTiles use kTmBISmall-row tiles: twice the workgroups, each with a
+// shorter chain, for grids that would leave most of the GPU idle.
+constexpr int kTmBISmall = 8;
+constexpr int kTmSmallTiles = 1024;
 constexpr int kTmBJ = 64;  // one wavefront per tile row
 __host__ __device__ constexpr int tm_halo(int order) { return order == 8 ? 7 : (order == 4 ? 3 : 1); }
 

@@ -23,13 +23,13 @@ constexpr int NTA = DGX_TM_ADJ_THREADS;  // threads per workgroup, adjoint
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-template <int ORDER>
+template <int ORDER, int TBI = BI>
 struct Tile {
     static constexpr int R = Radius<ORDER>::value;
-    static constexpr int HE = 2 * R - 1;       // halo of the staged Ez / lambda_Ez
-    static constexpr int LR = BI + 2 * HE;     // staged rows
-    static constexpr int LC = BJ + 2 * HE;     // staged columns
-    static constexpr int HR = BI + 2 * R - 1;  // rows of the Hy strip: R above the tile, R-1 below
+    static constexpr int HE = 2 * R - 1;        // halo of the staged Ez / lambda_Ez
+    static constexpr int LR = TBI + 2 * HE;     // staged rows
+    static constexpr int LC = BJ + 2 * HE;      // staged columns
+    static constexpr int HR = TBI + 2 * R - 1;  // rows of the Hy strip: R above the tile, R-1 below
     static constexpr int HC = BJ + 2 * R - 1;  // columns of the Hx strip
     static_assert(HE == tm_halo(ORDER), "halo width");
 };
@@ -120,11 +120,13 @@ __device__ __forceinline__ Cpml4 cpml_at(const float* R, int p, int q)
 // ---------------------------------------------------------------------------
 // The time step is it_off, plus *it_base when it_base is set (a replayed
 // graph of several steps reads the step of its first node from the device).
-template <int ORDER>
+// TBI: tile rows (g.bi).
+template <int ORDER, int TBI>
 __global__ __launch_bounds__(NTF) void k_tm_step_fwd(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
                                                      TmReceiversFwd rx, HistWrite hist, int it_off,
                                                      const int* __restrict__ it_base)
 {
+    constexpr int BI = TBI;
     const int tile = step_tile(g.ftiles);
     if (tile < 0) return;
     const int s = blockIdx.y;
@@ -173,10 +175,11 @@ __device__ __forceinline__ void reduce_terms(const Tm2dGrid& g, const Box& b, co
     if (want_sig) gsig[out] += sum_shots<ACC>(terms + sstride + h, b.cells, g.nshot);
 }
 
-template <int ORDER, typename ACC>
+template <int ORDER, typename ACC, int TBI>
 __global__ __launch_bounds__(NTA) void k_tm_step_adj(Tm2dGrid g, Tm2dFields f, TmPingPong pp, SourceBlocks src,
                                                      TmReceiversAdj rx, TmAdjGrad ag, int it)
 {
+    constexpr int BI = TBI;
     const int tile = step_tile(g.ftiles);
     if (tile < 0) return;
     const int s = blockIdx.y;
@@ -418,16 +421,25 @@ void launch_scale(const float* in, float* out, float factor, long long count, hi
 
 static dim3 step_grid(const Tm2dGrid& g) { return dim3((unsigned)((g.ftiles + 7) / 8 * 8), (unsigned)g.nshot); }
 
+template <int TBI>
+static void launch_fwd_rows(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
+                            const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
+                            hipStream_t st, const int* it_base)
+{
+    const dim3 grid = step_grid(g);
+    if (order == 8)
+        hipLaunchKernelGGL((k_tm_step_fwd<8, TBI>), grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
+    else if (order == 4)
+        hipLaunchKernelGGL((k_tm_step_fwd<4, TBI>), grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
+    else hipLaunchKernelGGL((k_tm_step_fwd<2, TBI>), grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
+}
+
 void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
                         const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
                         hipStream_t st, const int* it_base)
 {
-    const dim3 grid = step_grid(g);
-    if (order == 8)
-        hipLaunchKernelGGL(k_tm_step_fwd<8>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
-    else if (order == 4)
-        hipLaunchKernelGGL(k_tm_step_fwd<4>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
-    else hipLaunchKernelGGL(k_tm_step_fwd<2>, grid, dim3(NTF), 0, st, g, f, pp, src, rx, hw, it, it_base);
+    if (g.bi == kTmBISmall) launch_fwd_rows<kTmBISmall>(order, g, f, pp, src, rx, hw, it, st, it_base);
+    else launch_fwd_rows<kTmBI>(order, g, f, pp, src, rx, hw, it, st, it_base);
 }
 
 __global__ void k_step_counter(int* __restrict__ it, int value, int add)
@@ -440,24 +452,28 @@ void launch_step_counter(int* it, int value, bool add, hipStream_t st)
     hipLaunchKernelGGL(k_step_counter, dim3(1), dim3(64), 0, st, it, value, add ? 1 : 0);
 }
 
-template <typename ACC>
+template <typename ACC, int TBI>
 static void launch_adj_acc(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
                            const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it,
                            hipStream_t st)
 {
     const dim3 grid = step_grid(g);
-    if (order == 8) hipLaunchKernelGGL((k_tm_step_adj<8, ACC>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+    if (order == 8)
+        hipLaunchKernelGGL((k_tm_step_adj<8, ACC, TBI>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
     else if (order == 4)
-        hipLaunchKernelGGL((k_tm_step_adj<4, ACC>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
-    else hipLaunchKernelGGL((k_tm_step_adj<2, ACC>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+        hipLaunchKernelGGL((k_tm_step_adj<4, ACC, TBI>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
+    else hipLaunchKernelGGL((k_tm_step_adj<2, ACC, TBI>), grid, dim3(NTA), 0, st, g, f, pp, src, rx, ag, it);
 }
 
 void launch_tm_step_adj(int order, const Tm2dGrid& g, const Tm2dFields& f, const TmPingPong& pp,
                         const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it, bool fp64,
                         hipStream_t st)
 {
-    if (fp64) launch_adj_acc<double>(order, g, f, pp, src, rx, ag, it, st);
-    else launch_adj_acc<float>(order, g, f, pp, src, rx, ag, it, st);
+    const bool small = g.bi == kTmBISmall;
+    if (fp64 && small) launch_adj_acc<double, kTmBISmall>(order, g, f, pp, src, rx, ag, it, st);
+    else if (fp64) launch_adj_acc<double, kTmBI>(order, g, f, pp, src, rx, ag, it, st);
+    else if (small) launch_adj_acc<float, kTmBISmall>(order, g, f, pp, src, rx, ag, it, st);
+    else launch_adj_acc<float, kTmBI>(order, g, f, pp, src, rx, ag, it, st);
 }
 
 void launch_tm_grad_reduce(const Tm2dGrid& g, const Box& b, const float* terms, int want_eps, int want_sig,

@@ -143,13 +143,19 @@ def _tm2d_problem(h, NX, NY, order, seed, nsrc=2, nrx=4):
     return p
 
 
+# tile rows of the fused step kernels: 8 (chosen for small solves) and 16
+TILE_ROWS = ["8", "16"]
+
+
+@pytest.mark.parametrize("rows", TILE_ROWS)
 @pytest.mark.parametrize("order", [2, 4, 8])
-def test_tm2d_large_grid_matches_oracle(oracle_built, cuda_device, order):
+def test_tm2d_large_grid_matches_oracle(oracle_built, cuda_device, monkeypatch, order, rows):
     """A grid with tiles whose staged region (tile and halo) avoids the CPML
     and the grid faces, with sources and receivers inside such tiles and on
     tile edges."""
     h = oracle_built
-    print(f"[tm2d large] order={order}", flush=True)
+    monkeypatch.setenv("DEEPGPR_AMD_TM2D_TILE_ROWS", rows)
+    print(f"[tm2d large] order={order} rows={rows}", flush=True)
     p = _tm2d_problem(h, 90, 210, order, seed=60 + order, nsrc=3, nrx=6)
     for s in range(2):
         p.src_loc[s, 0] = [20 + 5 * s, 70, 0]
@@ -170,11 +176,13 @@ def test_tm2d_large_grid_matches_oracle(oracle_built, cuda_device, order):
     assert np.array_equal(gpu["E_saved"], cpu["E_saved"])
 
 
+@pytest.mark.parametrize("rows", TILE_ROWS)
 @pytest.mark.parametrize("order", [2, 4, 8])
 @pytest.mark.parametrize("mode,S,st", [(2, 1, "e_only"), (3, 1, "e_only"), (3, 2, "f32"), (3, 1, "bf16")])
-def test_tm2d_fast_path_matches_general_oracle(oracle_built, cuda_device, order, mode, S, st):
+def test_tm2d_fast_path_matches_general_oracle(oracle_built, cuda_device, monkeypatch, order, mode, S, st, rows):
     h = oracle_built
-    print(f"[tm2d] order={order} mode={mode} S={S} {st}", flush=True)
+    monkeypatch.setenv("DEEPGPR_AMD_TM2D_TILE_ROWS", rows)
+    print(f"[tm2d] order={order} mode={mode} S={S} {st} rows={rows}", flush=True)
     p = _tm2d_problem(h, 26, 23, order, seed=40 + order)
     p.mode, p.S = mode, S
     p.storage_type = _storage(h, st)
