@@ -4,9 +4,11 @@ on one GPU.
 Requests are admitted as slots and HBM free up (continuous batching). 2D
 Ez-TM requests join one running batch (``batch.BatchSolver``): every time
 step of every such request in flight is one kernel launch per FDTD order and
-phase, and a request joins or leaves between two groups of steps. Other
-requests (3D, INT8 histories, ...) each run on a HIP stream of their own, so
-their kernels overlap on the device. A request's state (fields, CPML
+phase, and a request joins or leaves between two groups of steps. By default
+they do so only while no request of another kind is waiting or running, and
+take a stream otherwise (``batch_policy``). Other requests (3D, INT8
+histories, ...) each run on a HIP stream of their own, so their kernels
+overlap on the device. A request's state (fields, CPML
 auxiliaries, histories) stays resident in HBM for the life of the request.
 Admission prices each request with ``estimate_compute_memory`` (the byte
 model of ``src/DeepGPR/solver/memory.py:30-268``); a request that does not fit
@@ -137,8 +139,14 @@ class SolveServer:
     ``batch_slots``: 2D Ez-TM requests in the running batch at once (0 turns
     the batch off; it is off with a custom ``runner`` or on the CPU).
     ``batch_chunk``: time steps between two points where requests join or
-    leave the batch. ``batch_solver``: factory ``f(device)`` of the object
-    that runs the batch (default ``batch.BatchSolver``). ``hbm_budget``: bytes the in-flight requests may hold
+    leave the batch. ``batch_policy``: ``"alone"`` (default) sends a
+    batchable request to the batch only while no request of another kind is
+    waiting or running, and to a stream otherwise; ``"always"`` sends every
+    batchable request to the batch. The batch is one chain of small dependent
+    kernels: beside the large kernels of other requests each of its steps
+    waits for free CUs, while on separate streams the small requests spread
+    that wait over several chains. ``batch_solver``: factory ``f(device)`` of
+    the object that runs the batch (default ``batch.BatchSolver``). ``hbm_budget``: bytes the in-flight requests may hold
     together (default: ``hbm_fraction`` of the free device memory when the
     server starts). ``window``: how far behind a request that does not fit the
     dispatcher looks for one that does.
@@ -147,7 +155,10 @@ class SolveServer:
     def __init__(self, device="cuda:0", slots: int = 8, hbm_budget: Optional[int] = None,
                  hbm_fraction: float = 0.8, window: int = 8,
                  runner: Optional[Callable[[SolveRequest, torch.device], tuple]] = None,
-                 batch_slots: int = 64, batch_chunk: int = 32, batch_solver: Optional[Callable] = None):
+                 batch_slots: int = 64, batch_chunk: int = 32, batch_solver: Optional[Callable] = None,
+                 batch_policy: str = "alone"):
+        if batch_policy not in ("alone", "always"):
+            raise ValueError(f"batch_policy must be 'alone' or 'always', not {batch_policy!r}")
         self.device = torch.device(device)
         if hbm_budget is None:
             hbm_budget = int(hbm_fraction * torch.cuda.mem_get_info(self.device)[0])
@@ -159,6 +170,8 @@ class SolveServer:
         use_batch = batch_slots > 0 and (batch_solver is not None or (runner is None and self.device.type == "cuda"))
         self.batch_slots = int(batch_slots) if use_batch else 0
         self.batch_chunk = int(batch_chunk)
+        self.batch_policy = batch_policy
+        self._others = 0  # requests the batch cannot serve, waiting or running
         self._pending: collections.deque = collections.deque()
         self._cv = threading.Condition()
         self._reserved = 0
@@ -187,6 +200,7 @@ class SolveServer:
         with self._cv:
             if self._closing:
                 raise RuntimeError("server is closed")
+            self._others += 0 if self._batchable(req) else 1
             self._pending.append((req, fut, need, time.perf_counter()))
             self._cv.notify_all()
         return fut
@@ -207,9 +221,15 @@ class SolveServer:
         self.close()
 
     # ------------------------------------------------------------ internals
-    def _to_batch(self, req: SolveRequest) -> bool:
+    @staticmethod
+    def _batchable(req: SolveRequest) -> bool:
         from .batch import batchable
-        return self.batch_slots > 0 and batchable(req)
+        return batchable(req)
+
+    def _to_batch(self, req: SolveRequest) -> bool:
+        if self.batch_slots <= 0 or not self._batchable(req):
+            return False
+        return self.batch_policy == "always" or self._others == 0
 
     def _pick(self):
         """(index, batched) of the next admissible pending request, or None."""
@@ -247,13 +267,14 @@ class SolveServer:
                 self._running += 1
             self._pool.submit(self._run, item)
 
-    def _release(self, need: int, batched: bool) -> None:
+    def _release(self, need: int, batched: bool, other: bool = False) -> None:
         with self._cv:
             self._reserved -= need
             if batched:
                 self._batched -= 1
             else:
                 self._running -= 1
+            self._others -= 1 if other else 0
             self._cv.notify_all()
 
     def _batch_loop(self) -> None:
@@ -339,7 +360,7 @@ class SolveServer:
         except BaseException as exc:  # noqa: BLE001 - delivered to the caller
             fut.set_exception(exc)
         finally:
-            self._release(need, False)
+            self._release(need, False, not self._batchable(req))
 
 
 def latency_stats(results: Sequence[SolveResult], wall: float) -> Dict:
@@ -357,12 +378,13 @@ def latency_stats(results: Sequence[SolveResult], wall: float) -> Dict:
 
 
 def run_stream(requests: Sequence[SolveRequest], device="cuda:0", slots: int = 8, paced: bool = True,
-               hbm_budget: Optional[int] = None, batch_slots: int = 64, batch_chunk: int = 32) -> Dict:
+               hbm_budget: Optional[int] = None, batch_slots: int = 64, batch_chunk: int = 32,
+               batch_policy: str = "alone") -> Dict:
     """Feeds requests to a server at their arrival times (or all at once when
     ``paced`` is False); returns the stats and the results by request id."""
     results: Dict[int, SolveResult] = {}
     with SolveServer(device, slots=slots, hbm_budget=hbm_budget, batch_slots=batch_slots,
-                     batch_chunk=batch_chunk) as server:
+                     batch_chunk=batch_chunk, batch_policy=batch_policy) as server:
         t0 = time.perf_counter()
         futs = []
         for req in requests:
@@ -416,6 +438,8 @@ def _main() -> int:
     ap.add_argument("--scale", type=float, default=1.0, help="multiplies every nt (shorter runs)")
     ap.add_argument("--batch-slots", type=int, default=64, help="2D requests in the running batch (0: off)")
     ap.add_argument("--batch-chunk", type=int, default=32, help="time steps between joins")
+    ap.add_argument("--batch-policy", default="alone", choices=("alone", "always"),
+                    help="alone: 2D requests join the batch only while no other kind is in the server")
     ap.add_argument("--streams-only", action="store_true", help="also run the server with the batch off")
     args = ap.parse_args()
     reqs = synthetic.serving_stream(args.requests, rate=args.rate, seed=args.seed, nt_scale=args.scale)
@@ -424,7 +448,8 @@ def _main() -> int:
     warm = synthetic.serving_stream(1, rate=args.rate, seed=args.seed + 1, nt_scale=0.05)
     run_sequential(warm, dev)
     seq = run_sequential(reqs, dev)
-    kw = dict(slots=args.slots, batch_slots=args.batch_slots, batch_chunk=args.batch_chunk)
+    kw = dict(slots=args.slots, batch_slots=args.batch_slots, batch_chunk=args.batch_chunk,
+              batch_policy=args.batch_policy)
     paced = run_stream(reqs, dev, paced=True, **kw)
     burst = run_stream(reqs, dev, paced=False, **kw)
     worst = 0.0
@@ -433,6 +458,7 @@ def _main() -> int:
         worst = max(worst, float((r.receivers - ref.receivers).abs().max()))
     line = {"metric": "serve p50/p99 ms vs plain solve", "requests": len(reqs), "rate_per_s": args.rate,
             "slots": args.slots, "batch_slots": args.batch_slots, "batch_chunk": args.batch_chunk,
+            "batch_policy": args.batch_policy,
             "nt_scale": args.scale, "sequential": seq["stats"], "served_paced": paced["stats"],
             "served_saturated": burst["stats"],
             "throughput_ratio": round(burst["stats"]["gcell_updates_per_s"] / seq["stats"]["gcell_updates_per_s"], 3),

@@ -117,7 +117,7 @@ def test_batchable_requests_share_the_batch_and_the_rest_take_streams(fail_rid):
             r.options = {"model_gradient_sampling_interval": 3}  # not batchable: stream path
         reqs.append(r)
     with serving.SolveServer("cpu", slots=2, hbm_budget=1 << 40, runner=fake, batch_slots=6, batch_chunk=8,
-                             batch_solver=lambda dev: _FakeBatch(dev, fail_rid)) as srv:
+                             batch_solver=lambda dev: _FakeBatch(dev, fail_rid), batch_policy="always") as srv:
         futs = [srv.submit(r) for r in reqs]
         done = concurrent.futures.wait(futs, timeout=30)
         assert not done.not_done
@@ -176,6 +176,29 @@ def test_request_larger_than_budget_fails():
             srv.submit(_req(0)).result(timeout=10)
 
 
+def test_batch_policy_alone_streams_2d_requests_beside_others():
+    """With the default policy a 2D request takes a stream while a 3D request
+    is waiting or running, and joins the batch once the server holds none."""
+    _FakeBatch.made.clear()
+    fake = _Fake(hold=0.05)
+    first = [_req3d(0)] + [_req(k, 128, nt=40) for k in range(1, 5)]
+    with serving.SolveServer("cpu", slots=8, hbm_budget=1 << 40, runner=fake, batch_slots=6, batch_chunk=8,
+                             batch_solver=lambda dev: _FakeBatch(dev)) as srv:
+        assert srv.batch_policy == "alone"
+        futs = [srv.submit(r) for r in first]
+        assert not concurrent.futures.wait(futs, timeout=30).not_done
+        later = [_req(k, 128, nt=40) for k in range(5, 9)]
+        futs += [srv.submit(r) for r in later]
+        assert not concurrent.futures.wait(futs, timeout=30).not_done
+    (batch,) = _FakeBatch.made
+    assert sorted(fake.order) == [0, 1, 2, 3, 4]
+    assert sorted(batch.seen) == [5, 6, 7, 8]
+    for r, f in zip(first + later, futs):
+        assert float(f.result().receivers[0, 0, 0]) == r.rid
+    with pytest.raises(ValueError):
+        serving.SolveServer("cpu", runner=fake, batch_policy="sometimes")
+
+
 def test_serving_stream_mix():
     reqs = synthetic.serving_stream(200, rate=50.0, seed=5051, nt_scale=0.01)
     kinds = [("3d" if r.eps_r.ndim == 3 else "2d", r.gradient) for r in reqs]
@@ -188,8 +211,8 @@ def test_serving_stream_mix():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("batch_slots", [0, 3])
-def test_results_same_alone_and_in_the_mix(cuda_device, batch_slots):
+@pytest.mark.parametrize("batch_slots,policy", [(0, "alone"), (3, "always"), (3, "alone")])
+def test_results_same_alone_and_in_the_mix(cuda_device, batch_slots, policy):
     reqs = synthetic.serving_stream(10, rate=200.0, seed=7, nt_scale=0.05)
     reqs = [r for r in reqs if r.eps_r.ndim == 2][:6]
     reqs[1].gradient = True
@@ -199,7 +222,7 @@ def test_results_same_alone_and_in_the_mix(cuda_device, batch_slots):
     reqs[-1].rid = 99
     alone = serving.run_sequential(reqs, cuda_device)["results"]
     mixed = serving.run_stream(reqs, cuda_device, slots=2, paced=False, batch_slots=batch_slots,
-                               batch_chunk=5)["results"]
+                               batch_chunk=5, batch_policy=policy)["results"]
     for r in reqs:
         a, m = alone[r.rid], mixed[r.rid]
         assert torch.equal(a.receivers, m.receivers), r.rid
