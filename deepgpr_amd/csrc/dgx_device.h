@@ -324,22 +324,30 @@ __device__ __forceinline__ int diff_slot(int comp, int axis)
     return axis == 0 ? (comp == 2 ? 3 : 4) : (axis == 1 ? (comp == 2 ? 0 : 5) : (comp == 1 ? 1 : 2));
 }
 
-// Cells of one launch of a cell sweep: every row j of the plane, and in each
-// row the k of two runs [za, za+na) and [zb, zb+nb). base numbers the
-// launch's workgroups after those of the launches before it.
-struct ZRuns {
+// Cells of a cell sweep: every row j of the plane, and in each row the k of
+// two runs [za, za+na) and [zb, zb+nb). A sweep is one launch of up to two
+// such parts: workgroups with blockIdx.x < split take part 0 (the z-interior
+// of every row), the others part 1 (the z-CPML cells), so a wavefront never
+// mixes the two while both parts share one launch.
+struct ZRun {
     int za, na, zb, nb;
-    int base;
+};
+struct ZRuns {
+    ZRun run[2];
+    int split;
 };
 
 __device__ __forceinline__ bool run_cell(const Grid& g, const ZRuns& zr, int& j, int& k)
 {
-    const int per = zr.na + zr.nb;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool second = (int)blockIdx.x >= zr.split;
+    const ZRun z = second ? zr.run[1] : zr.run[0];
+    const int bx = (int)blockIdx.x - (second ? zr.split : 0);
+    const int per = z.na + z.nb;
+    const int q = bx * (int)blockDim.x + (int)threadIdx.x;
     if (q >= per * g.N[1]) return false;
     j = q / per;
     const int r = q - j * per;
-    k = r < zr.na ? zr.za + r : zr.zb + (r - zr.na);
+    k = r < z.na ? z.za + r : z.zb + (r - z.na);
     return true;
 }
 

@@ -256,13 +256,12 @@ void launch_tm_material_gradient(const Tm2dGrid& g, const Tm2dFields& f, const M
 
 // The launches of a one-thread-per-cell sweep (see cell_sweep).
 struct CellSweep {
-    ZRuns run[2];
-    dim3 grid[2];
-    int count;
-    long long blocks;  // workgroups of all launches
+    ZRuns zr;
+    dim3 grid;
+    long long blocks;  // workgroups of the launch
 };
 CellSweep cell_sweep(const Grid& g, int order, bool split = true);
-// Workgroup (numbered across the launches) that advances cell (i, j, k) of shot s.
+// Workgroup (linear index in the launch) that advances cell (i, j, k) of shot s.
 long long cell_block(const CellSweep& cs, const Grid& g, int i, int j, int k, int s);
 void launch_coefficients(const Grid& g, const float* eps, const float* sig, const float* mu, float* const co[6],
                          float dt, hipStream_t st);

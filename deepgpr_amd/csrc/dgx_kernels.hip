@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__
     const bool pml = has_cpml && in_any_face(g, true, c);
 
     // block-local source entries (sorted by block, then source order)
-    const int blin = zr.base + blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     int sb = 0, se = 0;
     if (src.count > 0) {
         sb = src.block_off[blin];
@@ -697,46 +697,47 @@ __global__ __launch_bounds__(256) void k_adj_gather_e_w(Grid g, const float* __r
 CellSweep cell_sweep(const Grid& g, int order, bool split)
 {
     // The z-interior of every row (no z-face CPML term, no clamped z
-    // difference) and the rest run as two launches, so that a wavefront of
-    // consecutive cells rarely mixes the interior and the CPML branches.
+    // difference) and the rest are the two parts of the launch, so that a
+    // wavefront of consecutive cells rarely mixes the interior and the CPML
+    // branches, while the CPML workgroups fill the CUs beside the interior
+    // ones instead of running as a second, latency-bound launch.
     const int R = order / 2, nz = g.N[2];
     int lo = std::max(R, g.pml[4] > 0 ? g.pml[4] + 1 : 0);
     int hi = nz - 1 - R;
     if (g.pml[5] > 0) hi = std::min(hi, nz - 2 - g.pml[5]);
     CellSweep cs{};
     if (!split || lo > hi || hi - lo + 1 < 64) {
-        cs.count = 1;
-        cs.run[0] = ZRuns{0, nz, 0, 0, 0};
+        cs.zr.run[0] = ZRun{0, nz, 0, 0};
+        cs.zr.run[1] = ZRun{0, 0, 0, 0};
     } else {
-        cs.count = 2;
-        cs.run[0] = ZRuns{lo, hi - lo + 1, 0, 0, 0};
-        cs.run[1] = ZRuns{0, lo, hi + 1, nz - 1 - hi, 0};
+        cs.zr.run[0] = ZRun{lo, hi - lo + 1, 0, 0};
+        cs.zr.run[1] = ZRun{0, lo, hi + 1, nz - 1 - hi};
     }
-    long long base = 0;
-    for (int q = 0; q < cs.count; ++q) {
-        const long long cells = (long long)(cs.run[q].na + cs.run[q].nb) * g.N[1];
-        unsigned bx = (unsigned)((cells + 255) / 256);
-        // keep the x-planes of one column of blocks on the same XCD (8 XCDs,
-        // round-robin block placement) so x-neighbours are served by its L2
-        if (bx >= 8) bx = (bx + 7) / 8 * 8;
-        cs.grid[q] = dim3(bx, (unsigned)g.N[0], (unsigned)g.nshot);
-        cs.run[q].base = (int)base;
-        base += (long long)bx * g.N[0] * g.nshot;
-    }
-    cs.blocks = base;
+    const auto blocks_of = [&](const ZRun& z) {
+        return (long long)((long long)(z.na + z.nb) * g.N[1] + 255) / 256;
+    };
+    const long long b0 = blocks_of(cs.zr.run[0]);
+    unsigned bx = (unsigned)(b0 + blocks_of(cs.zr.run[1]));
+    // keep the x-planes of one column of blocks on the same XCD (8 XCDs,
+    // round-robin block placement) so x-neighbours are served by its L2; the
+    // padding workgroups fall in part 1 and exit
+    if (bx >= 8) bx = (bx + 7) / 8 * 8;
+    cs.zr.split = (int)b0;
+    cs.grid = dim3(bx, (unsigned)g.N[0], (unsigned)g.nshot);
+    cs.blocks = (long long)bx * g.N[0] * g.nshot;
     return cs;
 }
 
 long long cell_block(const CellSweep& cs, const Grid& g, int i, int j, int k, int s)
 {
-    for (int q = 0; q < cs.count; ++q) {
-        const ZRuns& z = cs.run[q];
+    for (int q = 0; q < 2; ++q) {
+        const ZRun& z = cs.zr.run[q];
         int r = -1;
         if (k >= z.za && k < z.za + z.na) r = k - z.za;
         else if (k >= z.zb && k < z.zb + z.nb) r = z.na + (k - z.zb);
         if (r < 0) continue;
         const long long cell = (long long)j * (z.na + z.nb) + r;
-        return z.base + cell / 256 + (long long)cs.grid[q].x * (i + (long long)g.N[0] * s);
+        return (q ? cs.zr.split : 0) + cell / 256 + (long long)cs.grid.x * (i + (long long)g.N[0] * s);
     }
     return -1;
 }
@@ -752,9 +753,9 @@ void launch_coefficients(const Grid& g, const float* eps, const float* sig, cons
 void launch_fwd_h(int order, const Grid& g, const StepArgs& a, hipStream_t st)
 {
     const CellSweep cs = cell_sweep(g, order);
-    for (int q = 0; q < cs.count; ++q) {
-        const dim3 grid = cs.grid[q];
-        const ZRuns zr = cs.run[q];
+    {
+        const dim3 grid = cs.grid;
+        const ZRuns zr = cs.zr;
         if (order == 8)
             hipLaunchKernelGGL(k_fwd_h<8>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml,
                                zr);
@@ -770,9 +771,9 @@ void launch_fwd_h(int order, const Grid& g, const StepArgs& a, hipStream_t st)
 void launch_fwd_e(int order, const Grid& g, const StepArgs& a, hipStream_t st)
 {
     const CellSweep cs = cell_sweep(g, order);
-    for (int q = 0; q < cs.count; ++q) {
-        const dim3 grid = cs.grid[q];
-        const ZRuns zr = cs.run[q];
+    {
+        const dim3 grid = cs.grid;
+        const ZRuns zr = cs.zr;
         if (order == 8)
             hipLaunchKernelGGL(k_fwd_e<8>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
                                a.has_cpml, a.src, a.hist, zr);
@@ -855,45 +856,45 @@ void launch_add_f64(const double* a, float* out, long long count, hipStream_t st
 void launch_adj_weights(int order, bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st)
 {
     const CellSweep cs = cell_sweep(g, order, false);  // measured: the split does not pay here
-    for (int q = 0; q < cs.count; ++q) {
+    {
         if (electric)
-            hipLaunchKernelGGL(k_adj_weights<true>, cs.grid[q], dim3(256), 0, st, g, a.ccurl, a.cb, a.lEc, a.cpe, W,
-                               a.has_cpml, cs.run[q]);
+            hipLaunchKernelGGL(k_adj_weights<true>, cs.grid, dim3(256), 0, st, g, a.ccurl, a.cb, a.lEc, a.cpe, W,
+                               a.has_cpml, cs.zr);
         else
-            hipLaunchKernelGGL(k_adj_weights<false>, cs.grid[q], dim3(256), 0, st, g, a.hcurl, a.hrhs, a.lHc, a.cph,
-                               W, a.has_cpml, cs.run[q]);
+            hipLaunchKernelGGL(k_adj_weights<false>, cs.grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.lHc, a.cph,
+                               W, a.has_cpml, cs.zr);
     }
 }
 
 void launch_adj_gather_e(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st)
 {
     const CellSweep cs = cell_sweep(g, order);
-    for (int q = 0; q < cs.count; ++q) {
-        const dim3 grid = cs.grid[q];
-        if (order == 8) hipLaunchKernelGGL(k_adj_gather_e<8>, grid, dim3(256), 0, st, g, W, a.lH, cs.run[q]);
-        else if (order == 4) hipLaunchKernelGGL(k_adj_gather_e<4>, grid, dim3(256), 0, st, g, W, a.lH, cs.run[q]);
-        else hipLaunchKernelGGL(k_adj_gather_e<2>, grid, dim3(256), 0, st, g, W, a.lH, cs.run[q]);
+    {
+        const dim3 grid = cs.grid;
+        if (order == 8) hipLaunchKernelGGL(k_adj_gather_e<8>, grid, dim3(256), 0, st, g, W, a.lH, cs.zr);
+        else if (order == 4) hipLaunchKernelGGL(k_adj_gather_e<4>, grid, dim3(256), 0, st, g, W, a.lH, cs.zr);
+        else hipLaunchKernelGGL(k_adj_gather_e<2>, grid, dim3(256), 0, st, g, W, a.lH, cs.zr);
     }
 }
 
 void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float* W, hipStream_t st)
 {
     const CellSweep cs = cell_sweep(g, order);
-    for (int q = 0; q < cs.count; ++q) {
-        const dim3 grid = cs.grid[q];
-        if (order == 8) hipLaunchKernelGGL(k_adj_gather_h<8>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.run[q]);
+    {
+        const dim3 grid = cs.grid;
+        if (order == 8) hipLaunchKernelGGL(k_adj_gather_h<8>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.zr);
         else if (order == 4)
-            hipLaunchKernelGGL(k_adj_gather_h<4>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.run[q]);
-        else hipLaunchKernelGGL(k_adj_gather_h<2>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.run[q]);
+            hipLaunchKernelGGL(k_adj_gather_h<4>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.zr);
+        else hipLaunchKernelGGL(k_adj_gather_h<2>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.zr);
     }
 }
 
 void launch_adj_gather_e_w(int order, const Grid& g, const AdjArgs& a, const float* We, float* Wh, hipStream_t st)
 {
     const CellSweep cs = cell_sweep(g, order);
-    for (int q = 0; q < cs.count; ++q) {
-        const dim3 grid = cs.grid[q];
-        const ZRuns zr = cs.run[q];
+    {
+        const dim3 grid = cs.grid;
+        const ZRuns zr = cs.zr;
         if (order == 8)
             hipLaunchKernelGGL(k_adj_gather_e_w<8>, grid, dim3(256), 0, st, g, We, a.lH, zr, a.hcurl, a.hrhs, a.cph,
                                Wh, a.has_cpml);
