@@ -67,6 +67,29 @@ def test_gpu_adjoint_receivers_match_oracle(oracle_built, cuda_device, order, rx
     assert not bad, bad
 
 
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_gpu_sources_in_both_parts_of_a_sweep(oracle_built, cuda_device, order):
+    """z long enough that each cell sweep runs as a z-interior part and a
+    z-CPML part in one launch: sources in the low and high z-CPML and in the
+    z-interior, each found through the workgroup numbering of its part."""
+    h = oracle_built
+    print(f"[gpu parity] sweep parts order={order}", flush=True)
+    p = h.make_problem(12, 11, 80, order=order, nt=9, nshot=2, nsrc=3, nrx=3, seed=21, pml=(3, 2, 3, 2, 4, 3),
+                       d=(0.02, 0.017, 0.013))
+    for s in range(2):
+        p.src_loc[s, 0] = [5 + s, 5, 1 + s]
+        p.src_loc[s, 1] = [6, 4 + s, 77 - s]
+        p.src_loc[s, 2] = [4, 6, 40 + s]
+        p.rx_loc[s, 0] = [6, 6, 3]
+        p.rx_loc[s, 1] = [5, 5, 76]
+        p.rx_loc[s, 2] = [7, 5, 41]
+    cpu = h.run_cpu(p, "oracle")
+    gpu = h.run_gpu(p, api="dgx")
+    assert np.abs(cpu["rx"]).max() > 0
+    bad = h.bitwise_diff(gpu, cpu, h.KEYS_FWD + h.KEYS_BWD)
+    assert not bad, bad
+
+
 def test_gpu_int8_history_matches_oracle(oracle_built, cuda_device):
     h = oracle_built
     # 64-cell tiles (one wavefront per tile) and a 16-cell tile (the general kernel)
