@@ -27,5 +27,5 @@ def cuda_device():
     if not torch.cuda.is_available():
         pytest.skip("no HIP device visible")
     from deepgpr_amd.native.loader import get_library
-    get_library()  # a GPU box without the built library must fail, not skip
+    get_library()  # with a GPU visible, a missing built library must fail, not skip
     return torch.device("cuda:0")
