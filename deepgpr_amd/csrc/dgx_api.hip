@@ -487,18 +487,23 @@ bool adj3d_fused()
     return fused;
 }
 
-// Tile rows of a single fused TM solve: kTmBISmall when its kTmBI-row tiles
-// over all shots number fewer than kTmSmallTiles (too few workgroups to fill
-// the GPU, so the shorter per-workgroup chain of the smaller tile wins), else
-// kTmBI. DEEPGPR_AMD_TM2D_TILE_ROWS=8 or 16 forces one; it is read on every
-// call, so one process can run both.
+// Tile rows of a single fused TM solve. Small solves leave most of the GPU
+// idle and their time is the chain of one workgroup, which shorter tiles cut:
+// kTmBITiny when its kTmBITiny-row tiles over all shots fit in one generation
+// of resident workgroups (kTmTinyGroups), else kTmBISmall when the kTmBI-row
+// tiles number fewer than kTmSmallTiles, else kTmBI.
+// DEEPGPR_AMD_TM2D_TILE_ROWS=4, 8 or 16 forces one; it is read on every call,
+// so one process can run all three.
 int tm2d_tile_rows(const Plan& pl)
 {
     const char* v = std::getenv("DEEPGPR_AMD_TM2D_TILE_ROWS");
+    if (v && std::strcmp(v, "4") == 0) return kTmBITiny;
     if (v && std::strcmp(v, "8") == 0) return kTmBISmall;
     if (v && std::strcmp(v, "16") == 0) return kTmBI;
-    const long long tiles = (long long)((pl.g.N[0] + kTmBI - 1) / kTmBI) * ((pl.g.N[1] + kTmBJ - 1) / kTmBJ);
-    return tiles * pl.g.nshot < kTmSmallTiles ? kTmBISmall : kTmBI;
+    const long long cols = (pl.g.N[1] + kTmBJ - 1) / kTmBJ;
+    const auto tiles = [&](int bi) { return (long long)((pl.g.N[0] + bi - 1) / bi) * cols * pl.g.nshot; };
+    if (tiles(kTmBITiny) <= kTmTinyGroups) return kTmBITiny;
+    return tiles(kTmBI) < kTmSmallTiles ? kTmBISmall : kTmBI;
 }
 
 Tm2dGrid tm_grid_of(const Plan& pl, int bi)

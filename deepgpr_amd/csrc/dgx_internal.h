@@ -94,7 +94,7 @@ struct Tm2dGrid {
     int tiles;   // ceil(plane / 256)
     int ftiles;  // tiles of the fused step kernels: nti * ntj
     int ntj;     // kTmBJ-wide tile columns
-    int bi;      // tile rows of the fused step kernels: kTmBI, or kTmBISmall for small solves
+    int bi;      // tile rows of the fused step kernels: kTmBI, kTmBISmall or kTmBITiny
     int nshot;
     float d[3];
     int pml[4];  // x0, xm, y0, ym
@@ -130,9 +130,13 @@ struct Tm2dWeights {
 constexpr int kTmBI = DGX_TM_TILE_ROWS;
 // Single solves whose kTmBI-row tiles (all shots) number fewer than
 // kTmSmallTiles use kTmBISmall-row tiles: twice the workgroups, each with a
-// shorter chain, for grids that would leave most of the GPU idle.
+// shorter chain, for grids that would leave most of the GPU idle. When even
+// kTmBITiny-row tiles number at most kTmTinyGroups (one generation of
+// resident workgroups: 256 CUs x 3), those are used.
 constexpr int kTmBISmall = 8;
 constexpr int kTmSmallTiles = 1024;
+constexpr int kTmBITiny = 4;
+constexpr int kTmTinyGroups = 768;
 constexpr int kTmBJ = 64;  // one wavefront per tile row
 __host__ __device__ constexpr int tm_halo(int order) { return order == 8 ? 7 : (order == 4 ? 3 : 1); }
 

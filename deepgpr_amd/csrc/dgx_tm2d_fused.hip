@@ -438,7 +438,8 @@ void launch_tm_step_fwd(int order, const Tm2dGrid& g, const Tm2dFields& f, const
                         const SourceBlocks& src, const TmReceiversFwd& rx, const HistWrite& hw, int it,
                         hipStream_t st, const int* it_base)
 {
-    if (g.bi == kTmBISmall) launch_fwd_rows<kTmBISmall>(order, g, f, pp, src, rx, hw, it, st, it_base);
+    if (g.bi == kTmBITiny) launch_fwd_rows<kTmBITiny>(order, g, f, pp, src, rx, hw, it, st, it_base);
+    else if (g.bi == kTmBISmall) launch_fwd_rows<kTmBISmall>(order, g, f, pp, src, rx, hw, it, st, it_base);
     else launch_fwd_rows<kTmBI>(order, g, f, pp, src, rx, hw, it, st, it_base);
 }
 
@@ -469,11 +470,16 @@ void launch_tm_step_adj(int order, const Tm2dGrid& g, const Tm2dFields& f, const
                         const SourceBlocks& src, const TmReceiversAdj& rx, const TmAdjGrad& ag, int it, bool fp64,
                         hipStream_t st)
 {
-    const bool small = g.bi == kTmBISmall;
-    if (fp64 && small) launch_adj_acc<double, kTmBISmall>(order, g, f, pp, src, rx, ag, it, st);
-    else if (fp64) launch_adj_acc<double, kTmBI>(order, g, f, pp, src, rx, ag, it, st);
-    else if (small) launch_adj_acc<float, kTmBISmall>(order, g, f, pp, src, rx, ag, it, st);
-    else launch_adj_acc<float, kTmBI>(order, g, f, pp, src, rx, ag, it, st);
+    if (g.bi == kTmBITiny) {
+        if (fp64) launch_adj_acc<double, kTmBITiny>(order, g, f, pp, src, rx, ag, it, st);
+        else launch_adj_acc<float, kTmBITiny>(order, g, f, pp, src, rx, ag, it, st);
+    } else if (g.bi == kTmBISmall) {
+        if (fp64) launch_adj_acc<double, kTmBISmall>(order, g, f, pp, src, rx, ag, it, st);
+        else launch_adj_acc<float, kTmBISmall>(order, g, f, pp, src, rx, ag, it, st);
+    } else {
+        if (fp64) launch_adj_acc<double, kTmBI>(order, g, f, pp, src, rx, ag, it, st);
+        else launch_adj_acc<float, kTmBI>(order, g, f, pp, src, rx, ag, it, st);
+    }
 }
 
 void launch_tm_grad_reduce(const Tm2dGrid& g, const Box& b, const float* terms, int want_eps, int want_sig,

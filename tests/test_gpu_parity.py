@@ -166,8 +166,8 @@ def _tm2d_problem(h, NX, NY, order, seed, nsrc=2, nrx=4):
     return p
 
 
-# tile rows of the fused step kernels: 8 (chosen for small solves) and 16
-TILE_ROWS = ["8", "16"]
+# tile rows of the fused step kernels: 4 and 8 (chosen for small solves) and 16
+TILE_ROWS = ["4", "8", "16"]
 
 
 @pytest.mark.parametrize("rows", TILE_ROWS)
