@@ -30,7 +30,7 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3"))
+    ap.add_argument("--workload", default="c3", choices=("c1", "c2", "c3"))
     ap.add_argument("--nshot", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--storage", default="float32", choices=("float32", "bfloat16", "float16", "int8"))
@@ -40,7 +40,6 @@ def _parse():
                     help="shots per compute() call (0: all that fit in free HBM)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
-    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run a plain device copy of this many MiB first (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--cpu-baseline-child", default=None, help=argparse.SUPPRESS)
@@ -201,9 +200,39 @@ def _shots_per_call(w, args, free_bytes: int) -> int:
 # ----------------------------------------------------------------------------- CPU baseline
 
 
+# CPU sample of each workload: (shots, nt); None = the workload's own value.
+# C3's full run needs about 223 GB of host memory per shot for its FP32 E+R
+# history and hours of CPU time, so its sample is 40 steps of one shot.
+CPU_SAMPLE = {"c1": (1, None), "c2": (1, None), "c3": (1, 40)}
+
+
+def physical_cores() -> int:
+    """Physical cores among the CPUs this process may run on (lscpu's
+    CPU,CORE,SOCKET table against sched_getaffinity), 1 thread per core."""
+    allowed = os.sched_getaffinity(0)
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=30).stdout
+        cores = set()
+        for line in out.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = (int(x) if x else 0 for x in line.split(",")[:3])
+            if cpu in allowed:
+                cores.add((sock, core))
+        if cores:
+            return len(cores)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return len(allowed)
+
+
 def _cpu_baseline_child(spec: str) -> None:
-    """Runs in a subprocess: the reference CPU backend (or the oracle) on a
-    bounded sample of the workload; prints one JSON line."""
+    """Runs in a subprocess: the reference CPU backend (or the oracle when the
+    reference build is absent) on the workload's CPU sample; one warm-up run,
+    then the median of three timed runs of forward + backward. Prints one
+    JSON line."""
+    import statistics
+
     from deepgpr_amd import synthetic
     from oracle import harness as H
 
@@ -211,36 +240,39 @@ def _cpu_baseline_child(spec: str) -> None:
     w = synthetic.WORKLOADS[args["workload"]]() if args["workload"] != "c1" else synthetic.c1_small_2d()
     kind = "reference" if H.reference_available() else "port"
     which = "ref" if kind == "reference" else "oracle"
-
-    def problem(nt):
-        return H.workload_problem(w, 1, nt, H.RHS_FROM_E if w.sampling_interval == 1 else 0)
-
-    calib = min(40, w.nt)
-    t0 = time.perf_counter()
-    H.run_cpu(problem(calib), which)
-    per_step = (time.perf_counter() - t0) / calib
-    nt = int(max(calib, min(w.nt, args["budget"] / max(per_step, 1e-9))))
-    p = problem(nt)
-    t0 = time.perf_counter()
-    H.run_cpu(p, which)
-    elapsed = time.perf_counter() - t0
+    shots, nt = CPU_SAMPLE[args["workload"]]
+    nt = nt or w.nt
+    p = H.workload_problem(w, shots, nt, H.RHS_FROM_E if w.sampling_interval == 1 else 0)
+    runs = []
+    for r in range(1 + args["repeats"]):
+        t = {}
+        H.run_cpu(p, which, timing=t)
+        if r:
+            runs.append(t["seconds"])
+    elapsed = statistics.median(runs)
     cells = w.extended_cells()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    print(json.dumps({"value": 2 * cells * nt / elapsed / 1e9, "unit": "Gcell-updates/s", "cores": threads,
-                      "kind": kind, "sample": f"{w.name}: shot 1 of {w.source_location.shape[0]}, nt={nt} of "
-                      f"{w.nt}, forward+adjoint, FP32 history, OpenMP threads={threads}, {elapsed:.2f} s"}))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    nshot = int(w.source_location.shape[0])
+    trunc = f", nt {nt} of {w.nt} (truncated)" if nt < w.nt else f", full nt {nt}"
+    print(json.dumps({"value": 2 * cells * nt * shots / elapsed / 1e9, "unit": "Gcell-updates/s",
+                      "cores": args["cores"], "threads": threads, "kind": kind,
+                      "sample": f"{w.name}: {shots} of {nshot} shots{trunc}, forward+adjoint, FP32 history, "
+                                f"OpenMP {threads} threads on {args['cores']} physical cores (close, cores), "
+                                f"1 warm-up + median of {len(runs)} runs: "
+                                + ", ".join(f"{x:.2f}" for x in runs) + " s"}))
 
 
 def _cpu_baseline(args):
-    spec = json.dumps({"workload": args.workload, "budget": args.cpu_budget_s})
+    cores = physical_cores()
+    spec = json.dumps({"workload": args.workload, "repeats": 3, "cores": cores})
     env = dict(os.environ)
-    env.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
-    env.setdefault("OMP_PROC_BIND", "close")
-    env.setdefault("OMP_PLACES", "cores")
+    env["OMP_NUM_THREADS"] = str(cores)
+    env["OMP_PROC_BIND"] = "close"
+    env["OMP_PLACES"] = "cores"
     res = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", spec],
-                         capture_output=True, text=True, env=env, timeout=600)
+                         capture_output=True, text=True, env=env, timeout=900)
     if res.returncode != 0:
-        return {"value": None, "unit": "Gcell-updates/s", "cores": None, "kind": None,
+        return {"value": None, "unit": "Gcell-updates/s", "cores": cores, "threads": cores, "kind": None,
                 "sample": "failed: " + res.stderr.strip()[-300:]}
     return json.loads(res.stdout.strip().splitlines()[-1])
 
@@ -284,10 +316,10 @@ def main():
         args.shots_per_call = _shots_per_call(w, args, torch.cuda.mem_get_info(dev)[0])
     chunk = args.shots_per_call
 
-    def step():
+    def step(first_call_only=False):
         e = eps.detach().requires_grad_(True)
         s = sig.detach().requires_grad_(True)
-        for c0 in range(0, nshot, chunk):
+        for c0 in range(0, chunk if first_call_only else nshot, chunk):
             out = dg.compute(dev, w.dx, w.dt, src_amp, src_loc[c0:c0 + chunk], rx_loc[c0:c0 + chunk], eps_r=e,
                              sigma=s, **kw)
             data = out[-1]
@@ -314,7 +346,7 @@ def main():
     if not args.no_kernel_timing:
         lib = get_library()
         lib.dgx_set_kernel_timing(1)
-        step()
+        step(first_call_only=True)  # per-launch averages: one compute() call is enough
         torch.cuda.synchronize(dev)
         lib.dgx_set_kernel_timing(0)
         for kid in range(64):
@@ -337,6 +369,8 @@ def main():
             roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": int(measured[0]) if measured else None,
+                        "traffic_kind": "profile constant (rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this kernel "
+                                        "and configuration, not of this run)" if measured else None,
                         "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_launch)}
             if measured:
                 roofline["traffic_source"] = measured[1]

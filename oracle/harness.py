@@ -336,8 +336,11 @@ def _history_buffers(p: Problem, real):
 
 
 def run_cpu(p: Problem, which: str = "oracle", real=np.float32, backward: bool = True,
-            threads: Optional[int] = None) -> Dict:
-    """Forward (+ adjoint) on the CPU: which = 'ref' | 'oracle'."""
+            threads: Optional[int] = None, timing: Optional[Dict] = None) -> Dict:
+    """Forward (+ adjoint) on the CPU: which = 'ref' | 'oracle'. With a
+    `timing` dict, the wall seconds of the forward and backward calls alone
+    (buffers allocated beforehand) are stored under "seconds"."""
+    import time
     if threads is not None:
         os.environ["OMP_NUM_THREADS"] = str(threads)
     lib = cpu_lib(which)
@@ -358,13 +361,18 @@ def run_cpu(p: Problem, which: str = "oracle", real=np.float32, backward: bool =
         err = getattr(lib, pre + "last_error")
     err.restype = ctypes.c_char_p
     fwd.restype = None
-    fwd(*_forward_args(p, st, real, E_saved, R_saved, coeffs, rdata), *extra)
+    fargs = _forward_args(p, st, real, E_saved, R_saved, coeffs, rdata)
+    t0 = time.perf_counter()
+    fwd(*fargs, *extra)
+    spent = time.perf_counter() - t0
     msg = err()
     if msg:
         raise RuntimeError(msg.decode())
     out = {"E": st["E"], "H": st["H"], "phi": st["phi"], "rx": rdata, "E_saved": E_saved, "R_saved": R_saved,
            "coeffs": coeffs}
     if not backward:
+        if timing is not None:
+            timing["seconds"] = spent
         return out
     st["L"] = [np.ascontiguousarray(x.astype(real)) for x in p.lam0]
     dgrad = np.ascontiguousarray(p.data_grad.astype(real))
@@ -374,7 +382,12 @@ def run_cpu(p: Problem, which: str = "oracle", real=np.float32, backward: bool =
     gsig = np.zeros(gshape, real)
     coeffs = [np.zeros((p.NX, p.NY, p.NZ), real) for _ in range(6)]
     bwd.restype = None
-    bwd(*_backward_args(p, st, real, E_saved, R_saved, coeffs, dgrad, gsrc, geps, gsig), *extra)
+    bargs = _backward_args(p, st, real, E_saved, R_saved, coeffs, dgrad, gsrc, geps, gsig)
+    t0 = time.perf_counter()
+    bwd(*bargs, *extra)
+    spent += time.perf_counter() - t0
+    if timing is not None:
+        timing["seconds"] = spent
     msg = err()
     if msg:
         raise RuntimeError(msg.decode())
