@@ -102,3 +102,40 @@ def test_cpu_device_is_refused():
         dg.compute("cpu", dx=0.02, dt=3e-11, source_amplitudes=torch.zeros(1, 10, 1),
                    source_location=torch.tensor([[[2, 2, 0]]]), receiver_location=torch.tensor([[[3, 3, 0]]]),
                    eps_r=torch.ones(8, 8), sigma=torch.zeros(8, 8), pmlthick=2)
+
+
+REF_HEADER = "/root/reference/src/DeepGPR/lib/deepgpr.h"
+
+
+def _prototypes(path, macro):
+    """name -> [(kind, parameter name)] of every `macro` declaration; kind is
+    the parameter's type with spaces normalised (const and pointers kept)."""
+    text = re.sub(r"/\*.*?\*/", "", open(path).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(macro + r"\s+([\w\s\*]+?)\b(\w+)\s*\(([^)]*)\)\s*;", text, flags=re.S):
+        params = m.group(3).strip()
+        plist = []
+        if params not in ("", "void"):
+            for p in params.split(","):
+                p = " ".join(p.replace("*", " * ").split())
+                *kind, name = p.split(" ")
+                plist.append((" ".join(kind).replace(" *", "*"), name))
+        out[m.group(2)] = (" ".join(m.group(1).replace("*", " * ").split()).replace(" *", "*"), plist)
+    return out
+
+
+@pytest.mark.skipif(not os.path.isfile(REF_HEADER), reason="reference sources not present")
+def test_abi6_exports_match_the_reference_header():
+    """Every export of the reference deepgpr.h (deepgpr.h:48-152) is declared
+    in include/deepgpr_amd.h with the same return kind and the same parameter
+    kinds, names and order, so the library binds as a drop-in deepgpr.so."""
+    ref = _prototypes(REF_HEADER, "DEEPGPR_API")
+    ours = _prototypes(HEADER, "DGX_API")
+    assert len(ref) >= 17
+    for name, (ret, params) in ref.items():
+        assert name in ours, name
+        oret, oparams = ours[name]
+        assert oret == ret, name
+        assert [k for k, _ in oparams] == [k for k, _ in params], name
+        assert [n for _, n in oparams] == [n for _, n in params], name
+    assert len(ref["forward"][1]) == 81 and len(ref["backward"][1]) == 86
