@@ -7,7 +7,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ("csrc/dgx_kernels.hip", "csrc/dgx_tm2d.hip", "csrc/dgx_tm2d_fused.hip", "csrc/dgx_api.hip")
+SOURCES = ("csrc/dgx_kernels.hip", "csrc/dgx_xmarch.hip", "csrc/dgx_tm2d.hip", "csrc/dgx_tm2d_fused.hip",
+           "csrc/dgx_api.hip")
 HIPCC_FLAGS = (
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
     # keep a*b+c unfused so the FP32 kernels reproduce the reference CPU bitwise

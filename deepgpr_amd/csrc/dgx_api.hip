@@ -12,6 +12,7 @@
 
 #include "deepgpr_amd.h"
 #include "dgx_internal.h"
+#include "dgx_retire.h"
 
 using namespace dgx;
 
@@ -85,7 +86,8 @@ long long g_timing_count[kKCount];
 const char* const kKernelNames[kKCount] = {
     "coefficients", "fwd_h", "fwd_e", "sample", "quant_int8", "save_frame", "adj_receivers",
     "adj_source", "material_gradient", "adj_pml_e", "adj_e", "adj_pml_h", "adj_h", "tm_step_fwd",
-    "tm_step_adj", "tm_grad_reduce", "tm_src_reduce", "adj_weights_e", "adj_weights_h", "adj_e_w"};
+    "tm_step_adj", "tm_grad_reduce", "tm_src_reduce", "adj_weights_e", "adj_weights_h", "adj_e_w",
+    "xm_fwd_h", "xm_fwd_e", "xm_adj_e", "xm_adj_h", "xm_psi_h"};
 
 struct LaunchTimer {
     bool on;
@@ -487,6 +489,15 @@ bool adj3d_fused()
     return fused;
 }
 
+// 3D (and other non-TM) solves run the one-thread-per-cell sweeps;
+// DEEPGPR_AMD_3D_SWEEP=xmarch selects the x-marching sweeps (same bits). Read
+// at every solve, so one process can run both.
+bool xmarch3d()
+{
+    const char* v = std::getenv("DEEPGPR_AMD_3D_SWEEP");
+    return v && std::strcmp(v, "xmarch") == 0;
+}
+
 // Tile rows of a single fused TM solve. Small solves leave most of the GPU
 // idle and their time is the chain of one workgroup, which shorter tiles cut:
 // kTmBITiny when its kTmBITiny-row tiles over all shots fit in one generation
@@ -811,9 +822,12 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     const bool inject = P->nsource > 0 && P->source_component >= 0 && P->source_component <= 2;
     const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
     const bool fused = tm && tm2d_fused();
+    const bool xm = !tm && xmarch3d();
+    const XmTiles xt = xm_tiles(g, pl.order);
     const Tm2dGrid tg = tm_grid_of(pl, tm2d_tile_rows(pl));
     const int tm_tiles = (g.N[0] * g.N[1] + 255) / 256;
-    long long nblocks = tm ? (long long)tm_tiles * g.nshot : sweep.blocks;
+    long long nblocks = tm ? (long long)tm_tiles * g.nshot
+                           : (xm ? (long long)xt.nty * xt.ntz * g.nshot : sweep.blocks);
     std::vector<int> boff((size_t)nblocks + 1, 0);
     std::vector<int2> ents;
     if (inject && fused) {
@@ -830,7 +844,9 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
             for (int k = 0; k < P->nsource; ++k) {
                 const int* L = &hsloc[((size_t)s * P->nsource + k) * 3];
                 long long m = (long long)L[0] * g.st0 + (long long)L[1] * g.N[2] + L[2];
-                long long blk = cell_block(sweep, g, L[0], L[1], L[2], s);
+                // x-marching sweeps: one list per (shot, tile), in plane order
+                long long blk = xm ? ((long long)s * xt.nty * xt.ntz + xm_tile_of(xt, L[1], L[2])) * g.N[0] + L[0]
+                                   : cell_block(sweep, g, L[0], L[1], L[2], s);
                 if (blk < 0 && !tm) {
                     set_error("forward: source cell outside the cell sweep");
                     return -1;
@@ -843,7 +859,8 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
             }
         std::stable_sort(tmp.begin(), tmp.end(), [](const E4& a, const E4& b) { return a.blk < b.blk; });
         for (const E4& e : tmp) {
-            boff[(size_t)e.blk + 1] += 1;
+            const long long list = xm ? e.blk / g.N[0] : e.blk;  // x-marching: key (list, plane)
+            boff[(size_t)list + 1] += 1;
             ents.push_back(make_int2(e.cell, e.src));
         }
         for (size_t b = 1; b < boff.size(); ++b) boff[b] += boff[b - 1];
@@ -925,6 +942,17 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     a.has_cpml = pl.has_cpml;
     a.src = src;
     const size_t eb = elem_bytes(pl.kind);
+    DevBuf d_kc;  // grid, CPML and coefficients of the x-marching sweeps
+    if (xm) {
+        const XmConst kc = xm_const(g, pl.cpe, pl.cph, co[0], co[1], co[2], co[4], co[5]);
+        if (!d_kc.alloc(sizeof(XmConst))) {
+            set_error("forward: out of device memory for the sweep constants");
+            return -1;
+        }
+        DGX_TRY(hipMemcpyAsync(d_kc.p, &kc, sizeof(XmConst), hipMemcpyHostToDevice, stream),
+                "forward: upload sweep constants");
+    }
+    const XmConst* kcp = (const XmConst*)d_kc.p;
     Tm2dState tms;
     if (tm && tms.setup(pl, P, stream, fused, tg.bi)) {
         set_error("forward: out of device memory for the 2D TM state");
@@ -1028,6 +1056,9 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         } else if (tm) {
             TIMED(tmr, kKFwdH, launch_tm_fwd_h(pl.order, tms.g, tms.f, stream));
             TIMED(tmr, kKFwdE, launch_tm_fwd_e(pl.order, tms.g, tms.f, a.src, a.hist, stream));
+        } else if (xm) {
+            TIMED(tmr, kKXmFwdH, launch_xm_fwd_h(pl.order, g, xt, kcp, a, stream));
+            TIMED(tmr, kKXmFwdE, launch_xm_fwd_e(pl.order, g, xt, kcp, a, stream));
         } else {
             TIMED(tmr, kKFwdH, launch_fwd_h(pl.order, g, a, stream));
             TIMED(tmr, kKFwdE, launch_fwd_e(pl.order, g, a, stream));
@@ -1196,7 +1227,9 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
         rg.ngroups = (int)gshot.size();
     }
 
-    const bool fuse3 = !tm && adj3d_fused();
+    const bool xm = !tm && xmarch3d();
+    const XmTiles xt = xm_tiles(g, pl.order);
+    const bool fuse3 = !tm && !xm && adj3d_fused();
 
     // fused 2D TM kernels: the groups again, listed for every (tile, shot)
     // whose staged region (tile plus halo) holds the cell
@@ -1267,7 +1300,7 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             tw.wh[f] = p;
             p += across * g.pml[f];
         }
-    } else if (!tm) {
+    } else if (!tm && !xm) {
         // six channel-weight arrays: shared by the E and H halves of the
         // adjoint, or one set per half when the gathers write the next weights
         const long long wlen = (fuse3 ? 12LL : 6LL) * g.nshot * g.cells;
@@ -1310,6 +1343,17 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     a.cpe = pl.cpe;
     a.cph = pl.cph;
     a.has_cpml = pl.has_cpml;
+    DevBuf d_kc;  // grid, CPML and coefficients of the x-marching sweeps
+    if (xm) {
+        const XmConst kc = xm_const(g, pl.cpe, pl.cph, co[0], co[1], co[2], co[4], co[5]);
+        if (!d_kc.alloc(sizeof(XmConst))) {
+            set_error("backward: out of device memory for the sweep constants");
+            return -1;
+        }
+        DGX_TRY(hipMemcpyAsync(d_kc.p, &kc, sizeof(XmConst), hipMemcpyHostToDevice, stream),
+                "backward: upload sweep constants");
+    }
+    const XmConst* kcp = (const XmConst*)d_kc.p;
 
     MatGrad mg{};
     mg.ca = co[0];
@@ -1371,6 +1415,9 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             TIMED(tmr, kKAdjE, launch_tm_adj_e(pl.order, tms.g, tms.f, tw, stream));
             if (a.has_cpml) TIMED(tmr, kKAdjPmlH, launch_tm_adj_pml(false, tms.g, tms.f, tw, stream));
             TIMED(tmr, kKAdjH, launch_tm_adj_h(pl.order, tms.g, tms.f, tw, stream));
+        } else if (xm) {
+            TIMED(tmr, kKXmAdjE, launch_xm_adj(pl.order, true, g, xt, kcp, a, it != nt - 1, stream));
+            TIMED(tmr, kKXmAdjH, launch_xm_adj(pl.order, false, g, xt, kcp, a, false, stream));
         } else if (fuse3) {
             float* We = (float*)d_w.p;
             float* Wh = We + 6LL * g.nshot * g.cells;
@@ -1385,6 +1432,7 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
             TIMED(tmr, kKAdjH, launch_adj_gather_h(pl.order, g, a, W, stream));
         }
     }
+    if (xm && nt > 0) TIMED(tmr, kKXmPsiH, launch_xm_psi_h_final(g, kcp, a, stream));
     if (tm && tms.unpack(P, stream)) {
         set_error("backward: copy back of the 2D TM state");
         return -1;
@@ -1490,10 +1538,21 @@ struct Staging {
 // recorded after its last reader has completed. Only then is it freed, so a
 // later stream-ordered allocation on any stream (instances are set up on
 // another stream) can never receive memory a queued step still uses.
-struct Retired {
+// Memory the batch stream may still read: a finished instance's buffers or a
+// replaced descriptor table (released by their destructors).
+struct Retiree {
     std::unique_ptr<BatchInstance> inst;
     std::unique_ptr<DevBuf> table;
-    hipEvent_t done = nullptr;
+};
+
+struct HipRetireOps {
+    using Event = hipEvent_t;
+    using Stream = hipStream_t;
+    bool create(Event* e) { return hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess; }
+    bool record(Event e, Stream s) { return hipEventRecord(e, s) == hipSuccess; }
+    bool done(Event e) { return hipEventQuery(e) == hipSuccess; }
+    void destroy(Event e) { (void)hipEventDestroy(e); }
+    bool sync(Stream s) { return hipStreamSynchronize(s) == hipSuccess; }
 };
 
 struct Batch {
@@ -1502,7 +1561,7 @@ struct Batch {
     int next_id = 1;
     std::vector<std::unique_ptr<BatchInstance>> active;
     std::vector<std::pair<int, hipEvent_t>> finished;
-    std::vector<Retired> retired;
+    RetireList<HipRetireOps, Retiree> retired;
     bool dirty = true;
     std::unique_ptr<DevBuf> table;  // descriptors and prefixes of the 6 slots
     size_t table_cap = 0;
@@ -1513,8 +1572,6 @@ struct Batch {
     int ninst[6] = {}, total[6] = {}, order_of[6] = {2, 4, 8, 2, 4, 8};
     ~Batch()
     {
-        for (Retired& r : retired)
-            if (r.done) (void)hipEventDestroy(r.done);
         for (Staging& r : ring) {
             if (r.done) (void)hipEventDestroy(r.done);
             if (r.host) (void)hipHostFree(r.host);
@@ -1525,31 +1582,19 @@ struct Batch {
 int order_slot(int order) { return order == 8 ? 2 : (order == 4 ? 1 : 0); }
 
 // Moves memory the batch stream may still read to the retired list, behind
-// an event recorded on the batch stream now (after every launch queued so far).
+// an event recorded on the batch stream now (after every launch queued so
+// far). When no event can be recorded the list drains the stream first, so
+// the memory is never released while queued steps may read it.
 int batch_retire(Batch* b, std::unique_ptr<BatchInstance> inst, std::unique_ptr<DevBuf> table)
 {
-    Retired r;
-    r.inst = std::move(inst);
-    r.table = std::move(table);
-    DGX_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming), "batch: event");
-    DGX_TRY(hipEventRecord(r.done, b->stream), "batch: event");
-    b->retired.push_back(std::move(r));
+    if (b->retired.retire(b->stream, Retiree{std::move(inst), std::move(table)}) != 0) {
+        set_error("batch: could not record a completion event for retired memory");
+        return -1;
+    }
     return 0;
 }
 
-// Frees retired memory whose last reader has finished (no waiting).
-void batch_sweep(Batch* b)
-{
-    for (size_t q = 0; q < b->retired.size();) {
-        Retired& r = b->retired[q];
-        if (hipEventQuery(r.done) != hipSuccess) {
-            ++q;
-            continue;
-        }
-        (void)hipEventDestroy(r.done);
-        b->retired.erase(b->retired.begin() + (long)q);  // frees the buffers
-    }
-}
+void batch_sweep(Batch* b) { b->retired.sweep(); }
 
 // Sets an instance up on `stream` (not the batch stream, so that the host
 // waits only for the set-up work); the batch stream waits for it before the
@@ -1952,7 +1997,12 @@ DGX_API long long dgx_scratch_bytes(const dgx_solve* solve, int backward)
     long long bytes = 0;
     if (!backward) {
         if (pl.history && pl.store_r && pl.kind == kInt8) bytes += (long long)pl.comps * pl.snap * 4;
-        bytes += (cell_sweep(g, pl.order).blocks + 1) * 4;
+        if (xmarch3d()) {
+            const XmTiles xt = xm_tiles(g, pl.order);
+            bytes += ((long long)xt.nty * xt.ntz * g.nshot + 1) * 4;
+        } else {
+            bytes += (cell_sweep(g, pl.order).blocks + 1) * 4;
+        }
     } else {
         if (pl.has_cpml)
             for (int f = 0; f < 6; ++f)
@@ -1961,7 +2011,7 @@ DGX_API long long dgx_scratch_bytes(const dgx_solve* solve, int backward)
         // adjoint channel weights of the general (non-2D-TM) path
         const bool tm_shape = (solve->storage_type & kFlagTm2d) && g.N[2] == 2 && solve->source_component == 2 &&
                               pl.kind != kInt8;
-        if (!tm_shape) bytes += (adj3d_fused() ? 48LL : 24LL) * g.nshot * g.cells;
+        if (!tm_shape && !xmarch3d()) bytes += (adj3d_fused() ? 48LL : 24LL) * g.nshot * g.cells;
     }
     return bytes;
 }

@@ -135,8 +135,8 @@ __device__ __forceinline__ float dfwd(const float* __restrict__ f, IDX id, typen
 
 // The unclamped differences (the full-radius branch of dbwd / dfwd), for
 // cells at least R away from every face.
-template <int ORDER>
-__device__ __forceinline__ float dbwd_full(const float* __restrict__ f, long long id, long long st)
+template <int ORDER, typename IDX = long long>
+__device__ __forceinline__ float dbwd_full(const float* __restrict__ f, IDX id, typename NoDeduce<IDX>::type st)
 {
     if constexpr (ORDER <= 2) {
         return f[id] - f[id - st];
@@ -149,8 +149,8 @@ __device__ __forceinline__ float dbwd_full(const float* __restrict__ f, long lon
     }
 }
 
-template <int ORDER>
-__device__ __forceinline__ float dfwd_full(const float* __restrict__ f, long long id, long long st)
+template <int ORDER, typename IDX = long long>
+__device__ __forceinline__ float dfwd_full(const float* __restrict__ f, IDX id, typename NoDeduce<IDX>::type st)
 {
     if constexpr (ORDER <= 2) {
         return f[id + st] - f[id];

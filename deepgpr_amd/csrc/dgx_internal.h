@@ -291,11 +291,52 @@ void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float
 // the E-side gather fused with the H-side weights: W_E -> lambda_H -> W_H
 void launch_adj_gather_e_w(int order, const Grid& g, const AdjArgs& a, const float* We, float* Wh, hipStream_t st);
 
+// x-marching 3D sweeps (dgx_xmarch.hip): a workgroup owns a (y, z) tile of
+// `ty` rows (one wavefront each) x 64 cells of one shot and walks `xc`
+// x-planes; tiles are numbered (y tile) * ntz + (z tile), in XCD-contiguous
+// launch order when `remap` is set.
+struct XmTiles {
+    int nty, ntz;
+    int ty;
+    int xc;
+    int remap;
+};
+// What the sweeps' out-of-line CPML code reads, in device memory (one copy per
+// solve): the grid, both CPML sets and the update coefficients.
+struct XmConst {
+    Grid g;
+    Cpml cpe, cph;
+    const float *ca, *ccurl, *cb, *hcurl, *hrhs;
+};
+XmConst xm_const(const Grid& g, const Cpml& cpe, const Cpml& cph, const float* ca, const float* ccurl,
+                 const float* cb, const float* hcurl, const float* hrhs);
+// Tile rows (4 or 8), planes per chunk and remap of the 3D sweeps;
+// DEEPGPR_AMD_XM_TILE_ROWS, DEEPGPR_AMD_XM_PLANES and DEEPGPR_AMD_XM_REMAP
+// override the defaults.
+int xm_tile_rows(int order);
+int xm_chunk_planes();
+int xm_remap();
+XmTiles xm_tiles(const Grid& g, int order);
+int xm_tile_of(const XmTiles& xt, int j, int k);
+void launch_xm_fwd_h(int order, const Grid& g, const XmTiles& xt, const XmConst* kc, const StepArgs& a,
+                     hipStream_t st);
+// sources: block_off indexed by shot * (nty * ntz) + tile, entries in plane
+// order and, within a plane, source order
+void launch_xm_fwd_e(int order, const Grid& g, const XmTiles& xt, const XmConst* kc, const StepArgs& a,
+                     hipStream_t st);
+// electric: lambda_H += D-^T W_E (with `defer`, first the lambda psi_H update
+// owed by the previous step); otherwise lambda_E = ca lambda_E + D+^T W_H,
+// advancing lambda psi_E
+void launch_xm_adj(int order, bool electric, const Grid& g, const XmTiles& xt, const XmConst* kc,
+                   const AdjArgs& a, bool defer, hipStream_t st);
+// the lambda psi_H update owed by the last adjoint step
+void launch_xm_psi_h_final(const Grid& g, const XmConst* kc, const AdjArgs& a, hipStream_t st);
+
 // Kernel ids of the optional per-launch timing (dgx_set_kernel_timing).
 enum KernelId : int {
     kKCoeff = 0, kKFwdH, kKFwdE, kKSample, kKQuant, kKSaveFrame, kKAdjRx, kKAdjSrc, kKMatGrad, kKAdjPmlE,
     kKAdjE, kKAdjPmlH, kKAdjH, kKTmStepFwd, kKTmStepAdj, kKTmGradReduce, kKTmSrcReduce, kKAdjWeightsE,
-    kKAdjWeightsH, kKAdjEW, kKCount
+    kKAdjWeightsH, kKAdjEW, kKXmFwdH, kKXmFwdE, kKXmAdjE, kKXmAdjH, kKXmPsiH, kKCount
 };
 
 }  // namespace dgx

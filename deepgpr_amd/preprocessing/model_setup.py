@@ -1,14 +1,19 @@
 """Validation of the physical model and its extension into the CPML
-(src/DeepGPR/preprocessing/model_setup.py:36-406).
+(the interface of src/DeepGPR/preprocessing/model_setup.py:36-406: the same
+arguments, exception types and messages).
 
 The caller supplies only the physical model; every solve replicates the edge
 materials outward into the CPML and pads the one-cell Yee halo. The adjoint of
 the replication folds the extended-grid sensitivities back onto the physical
 edges, faces and corners, one axis at a time.
+
+Design: the host-side shape checks run first; every check that needs the
+tensors' values is then one entry of a rule table (`_VALUE_RULES`) over named
+device scalars, all evaluated with a single device-to-host transfer.
 """
 from __future__ import annotations
 
-from typing import NamedTuple, Sequence, Tuple
+from typing import Callable, Dict, List, NamedTuple, Sequence, Tuple
 
 import torch
 import torch.nn.functional as F
@@ -46,8 +51,10 @@ class PreparedModel(NamedTuple):
 
 
 class ExtendModel(torch.autograd.Function):
-    """Replicate-pad a material into the CPML; backward sums the replicas onto
-    their owner cells axis by axis (no scatter, deterministic)."""
+    """Replicate-pad a material into the CPML. The backward is the transpose
+    of the replication: along each axis in turn, the replicas on either side
+    are summed onto the edge cell they copy (no scatter, so the result does
+    not depend on scheduling)."""
 
     @staticmethod
     def forward(ctx, model: torch.Tensor, pml: Sequence[int]) -> torch.Tensor:  # type: ignore[override]
@@ -76,62 +83,105 @@ def shift_locations(locations: torch.Tensor, pmlthick: torch.Tensor, device) -> 
     return (locations.to(device=device, dtype=torch.int32) + offset).contiguous()
 
 
-def _as3d(t: torch.Tensor, what: str) -> torch.Tensor:
-    if t.ndim == 2:
-        return t.reshape(*t.shape, 1)
-    if t.ndim != 3:
+# ----------------------------------------------------------------------------- host-side checks
+
+
+def _grid3(t: torch.Tensor, what: str) -> torch.Tensor:
+    """A 2D model gains a unit z axis; anything but 2D or 3D is refused."""
+    if t.ndim not in (2, 3):
         raise ValueError(f"The shape of {what} should be 2-d or 3-d.")
-    return t
+    return t.unsqueeze(-1) if t.ndim == 2 else t
 
 
-def _check_locations(src: torch.Tensor, rx: torch.Tensor, device: torch.device) -> Tuple[int, int, int]:
-    if not torch.is_tensor(src) or not torch.is_tensor(rx):
+_LOCATION_SHAPE_RULES: List[Tuple[Callable[[torch.Tensor, torch.Tensor], bool], str]] = [
+    (lambda s, r: s.ndim == 3 and r.ndim == 3, "source_location and receiver_location must have shape (nstep, count, 3)."),
+    (lambda s, r: s.shape[2] == 3 and r.shape[2] == 3,
+     "The last dimension of source_location and receiver_location must be 3."),
+    (lambda s, r: s.shape[0] == r.shape[0],
+     "The first dimension (nstep) of source_location and receiver_location should be the same."),
+    (lambda s, r: s.shape[0] >= 1, "At least one shot is required."),
+    (lambda s, r: s.shape[1] >= 1, "At least one source per shot is required."),
+    (lambda s, r: r.shape[1] >= 1, "At least one receiver per shot is required."),
+]
+
+
+def _location_counts(src, rx, device: torch.device) -> Tuple[int, int, int]:
+    """(shots, sources per shot, receivers per shot) of well-shaped location tensors."""
+    if not (torch.is_tensor(src) and torch.is_tensor(rx)):
         raise TypeError("source_location and receiver_location must be PyTorch tensors.")
-    if src.ndim != 3 or rx.ndim != 3:
-        raise ValueError("source_location and receiver_location must have shape (nstep, count, 3).")
-    if src.shape[2] != 3 or rx.shape[2] != 3:
-        raise ValueError("The last dimension of source_location and receiver_location must be 3.")
-    if src.shape[0] != rx.shape[0]:
-        raise ValueError("The first dimension (nstep) of source_location and receiver_location should be the same.")
-    nstep, nsr, nrx = src.shape[0], src.shape[1], rx.shape[1]
-    if nstep < 1:
-        raise ValueError("At least one shot is required.")
-    if nsr < 1:
-        raise ValueError("At least one source per shot is required.")
-    if nrx < 1:
-        raise ValueError("At least one receiver per shot is required.")
-    if device.type == "cuda" and nstep > CUDA_MAX_SHOTS_PER_CALL:
+    for holds, message in _LOCATION_SHAPE_RULES:
+        if not holds(src, rx):
+            raise ValueError(message)
+    if device.type == "cuda" and src.shape[0] > CUDA_MAX_SHOTS_PER_CALL:
         raise ValueError(f"At most {CUDA_MAX_SHOTS_PER_CALL} shots fit in one compute call; split the batch.")
     for name, t in (("source_location", src), ("receiver_location", rx)):
         if t.dtype == torch.bool or t.is_complex():
             raise TypeError(f"{name} must contain real integer-valued coordinates.")
-    return nstep, nsr, nrx
+    return int(src.shape[0]), int(src.shape[1]), int(rx.shape[1])
 
 
-def _waveforms(w: torch.Tensor, nsr: int, device, dtype) -> torch.Tensor:
+def _per_source_waveforms(w, nsr: int, device, dtype) -> torch.Tensor:
+    """(nsr, nt, 1) waveforms: one per source, or one shared by every source."""
     if not torch.is_tensor(w):
         raise TypeError("source_amplitudes must be a PyTorch tensor.")
     if w.ndim not in (2, 3):
         raise ValueError("source_amplitudes must have shape (num_waveforms, nt) or (num_waveforms, nt, 1).")
-    if w.ndim == 2:
-        w = w.unsqueeze(-1)
+    w = w if w.ndim == 3 else w[..., None]
     if w.shape[2] != 1:
         raise ValueError("The last dimension of source_amplitudes must be 1.")
-    if w.shape[0] < 1 or w.shape[1] < 1:
+    if min(w.shape[0], w.shape[1]) < 1:
         raise ValueError("source_amplitudes must contain at least one waveform and one time sample.")
-    w = w.to(device=device, dtype=dtype).contiguous()
-    n = w.shape[0]
-    if n > nsr or 1 < n < nsr:
+    if w.shape[0] not in (1, nsr):
         raise ValueError("The number of source waveforms is incorrect.")
-    if n == 1 and nsr != 1:
-        w = w.repeat(nsr, 1, 1).contiguous()
-    return w
+    w = w.to(device=device, dtype=dtype)
+    return (w.expand(nsr, -1, -1) if w.shape[0] != nsr else w).contiguous()
 
 
-def _integral(t: torch.Tensor) -> torch.Tensor:
-    if t.is_floating_point():
-        return (torch.isfinite(t) & (t == t.trunc())).all()
-    return torch.ones((), dtype=torch.bool, device=t.device)
+# ----------------------------------------------------------------------------- value checks
+
+
+def _whole(t: torch.Tensor) -> torch.Tensor:
+    """All entries are finite whole numbers (always true for integer tensors)."""
+    if not t.is_floating_point():
+        return torch.ones((), dtype=torch.bool, device=t.device)
+    return (torch.isfinite(t) & (torch.frac(t) == 0)).all()
+
+
+def _inside(loc: torch.Tensor, shape: torch.Tensor) -> torch.Tensor:
+    return ((loc >= 0) & (loc < shape)).all()
+
+
+# (name of a device scalar, whether its host value passes, exception, message);
+# checked in this order once all scalars have reached the host.
+_VALUE_RULES: List[Tuple[str, Callable[[float], bool], type, str]] = [
+    ("finite_er", bool, ValueError, "`er` contains NaN or Inf values."),
+    ("finite_se", bool, ValueError, "`se` contains NaN or Inf values."),
+    ("finite_mr", bool, ValueError, "`mr` contains NaN or Inf values."),
+    ("finite_wave", bool, ValueError, "`source_amplitudes` contains NaN or Inf values."),
+    ("min_er", lambda v: v >= MIN_RELATIVE_PERMITTIVITY, ValueError,
+     "The values of epsilon is incorrect.(should be greater than 1)"),
+    ("min_se", lambda v: v >= 0, ValueError, "The values of sigma is incorrect.(should be non-negative)"),
+    ("min_mr", lambda v: v > 0, ValueError, "The values of mr are incorrect (must be positive)."),
+    ("whole_src", bool, ValueError, "source_location must contain finite integer-valued coordinates."),
+    ("whole_rx", bool, ValueError, "receiver_location must contain finite integer-valued coordinates."),
+    ("inside_src", bool, ValueError, "Error: Source coordinates out of range! Valid ranges are {ranges}"),
+    ("inside_rx", bool, ValueError, "Error: Receiver coordinates out of range! Valid ranges are {ranges}"),
+]
+
+
+def _device_scalars(er, se, mr, wave, src, rx) -> Dict[str, float]:
+    """Every value the checks need, moved to the host in one transfer."""
+    shape = torch.tensor(er.shape, dtype=torch.int32, device=er.device)
+    named = {
+        "finite_er": torch.isfinite(er).all(), "finite_se": torch.isfinite(se).all(),
+        "finite_mr": torch.isfinite(mr).all(), "finite_wave": torch.isfinite(wave).all(),
+        "min_er": er.amin(), "min_se": se.amin(), "min_mr": mr.amin(),
+        "min_er_mr": (er.detach() * mr.detach()).amin(), "max_se": se.amax(),
+        "whole_src": _whole(src), "whole_rx": _whole(rx),
+        "inside_src": _inside(src, shape), "inside_rx": _inside(rx, shape),
+    }
+    values = torch.stack([v.detach().to(torch.float32) for v in named.values()]).cpu().tolist()
+    return dict(zip(named, values))
 
 
 def initialization(device, er, se, mr, source_amplitudes, source_location, receiver_location,
@@ -141,87 +191,51 @@ def initialization(device, er, se, mr, source_amplitudes, source_location, recei
     device = torch.device("cpu" if device is None else device)
     dtype = NATIVE_FLOAT_DTYPE
     spacing = normalize_grid_spacing(dx)
-    step = float(dt)
-    if not (step > 0.0 and step < float("inf")):
+    if not 0.0 < float(dt) < float("inf"):
         raise ValueError("dt must be a finite positive number.")
-    if not torch.is_tensor(er) or not torch.is_tensor(se):
+    if not (torch.is_tensor(er) and torch.is_tensor(se)):
         raise TypeError("er and se must be PyTorch tensors.")
-    er = _as3d(er, "epsilon")
-    se = _as3d(se, "sigma")
     if mr is not None and not torch.is_tensor(mr):
         raise TypeError("mr must be a PyTorch tensor or None.")
-    if mr is not None:
-        mr = _as3d(mr, "mr")
+    er, se = _grid3(er, "epsilon"), _grid3(se, "sigma")
+    mr = None if mr is None else _grid3(mr, "mr")
     if er.shape != se.shape:
         raise ValueError("The shape of epsilon and sigma should be the same.")
-    if any(n < 1 for n in er.shape):
+    if min(er.shape) < 1:
         raise ValueError("The material model dimensions must all be non-empty.")
-    nx, ny, nz = er.shape
-    mode = 2 if nz == 1 else 3
-    er = er.to(device=device, dtype=dtype)
-    se = se.to(device=device, dtype=dtype)
-    if mr is None:
-        mr = torch.ones_like(er, device=device)
-    elif mr.shape == er.shape:
-        mr = mr.to(device=device, dtype=dtype)
-    else:
+    if mr is not None and mr.shape != er.shape:
         raise ValueError("The shape of mr should be the same as epsilon and sigma.")
+    nx, ny, nz = (int(n) for n in er.shape)
+    er, se = er.to(device=device, dtype=dtype), se.to(device=device, dtype=dtype)
+    mr = torch.ones_like(er) if mr is None else mr.to(device=device, dtype=dtype)
 
-    nstep, nsr, nrx = _check_locations(source_location, receiver_location, device)
+    nstep, nsr, nrx = _location_counts(source_location, receiver_location, device)
     source_location = source_location.to(device=device).contiguous()
     receiver_location = receiver_location.to(device=device).contiguous()
-    source_amplitudes = _waveforms(source_amplitudes, nsr, device, dtype)
-    nt = source_amplitudes.shape[1]
+    source_amplitudes = _per_source_waveforms(source_amplitudes, nsr, device, dtype)
 
     pmlthick = pmlthick_revert(pmlthick, er)
     pml = [int(v) for v in pmlthick.tolist()]
-    if nz == 1 and any(pml[4:]):
+    if nz == 1 and (pml[4] or pml[5]):
         raise ValueError("2D models require zero PML thickness on the z boundaries.")
     ext = tuple(n + pml[2 * a] + pml[2 * a + 1] for a, n in enumerate((nx, ny, nz)))
-    if any(n >= INT32_MAX for n in ext):
+    if max(ext) >= INT32_MAX:
         raise ValueError("Extended model dimensions must fit in positive int32 with a field halo.")
 
-    # one device->host transfer for every scalar the validation needs
-    shape = torch.tensor((nx, ny, nz), dtype=torch.int32, device=device)
-    stats = torch.stack((
-        torch.isfinite(er).all().to(dtype), torch.isfinite(se).all().to(dtype),
-        torch.isfinite(mr).all().to(dtype), torch.isfinite(source_amplitudes).all().to(dtype),
-        er.amin(), se.amin(), mr.amin(), (er.detach() * mr.detach()).amin(),
-        ((source_location >= 0) & (source_location < shape)).all().to(dtype),
-        ((receiver_location >= 0) & (receiver_location < shape)).all().to(dtype),
-        _integral(source_location).to(dtype), _integral(receiver_location).to(dtype),
-        se.amax(),
-    )).detach().cpu().tolist()
-    for name, ok in zip(("er", "se", "mr", "source_amplitudes"), stats[:4]):
-        if not ok:
-            raise ValueError(f"`{name}` contains NaN or Inf values.")
-    if stats[4] < MIN_RELATIVE_PERMITTIVITY:
-        raise ValueError("The values of epsilon is incorrect.(should be greater than 1)")
-    if stats[5] < 0:
-        raise ValueError("The values of sigma is incorrect.(should be non-negative)")
-    if stats[6] <= 0:
-        raise ValueError("The values of mr are incorrect (must be positive).")
-    if not stats[10]:
-        raise ValueError("source_location must contain finite integer-valued coordinates.")
-    if not stats[11]:
-        raise ValueError("receiver_location must contain finite integer-valued coordinates.")
-    if not stats[8]:
-        raise ValueError(f"Error: Source coordinates out of range! Valid ranges are x∈[0,{nx}), "
-                         f"y∈[0,{ny}), z∈[0,{nz})")
-    if not stats[9]:
-        raise ValueError(f"Error: Receiver coordinates out of range! Valid ranges are x∈[0,{nx}), "
-                         f"y∈[0,{ny}), z∈[0,{nz})")
-    if stats[12] > PEC_CONDUCTIVITY_THRESHOLD:
+    v = _device_scalars(er, se, mr, source_amplitudes, source_location, receiver_location)
+    ranges = f"x∈[0,{nx}), y∈[0,{ny}), z∈[0,{nz})"
+    for name, passes, exc, message in _VALUE_RULES:
+        if not passes(v[name]):
+            raise exc(message.format(ranges=ranges))
+    if v["max_se"] > PEC_CONDUCTIVITY_THRESHOLD:
         _LOG.warning("Conductivity above %g S/m is treated as a perfect electric conductor.",
                      PEC_CONDUCTIVITY_THRESHOLD)
-    check_cfl(spacing, dt, *ext, fdtd_order=fdtd_order, _material_min_er_mr=stats[7])
+    check_cfl(spacing, dt, *ext, fdtd_order=fdtd_order, _material_min_er_mr=v["min_er_mr"])
     if any(pml):
-        er = ExtendModel.apply(er, pml)
-        se = ExtendModel.apply(se, pml)
-        mr = ExtendModel.apply(mr, pml)
-    nx, ny, nz = ext
-    return PreparedModel(er, se, nx, ny, nz, nt, nstep, nsr, nrx, F.pad(er, _HALO), F.pad(se, _HALO),
-                         F.pad(mr, _HALO), mode, dtype, pmlthick, source_amplitudes)
+        er, se, mr = (ExtendModel.apply(t, pml) for t in (er, se, mr))
+    return PreparedModel(er, se, *ext, source_amplitudes.shape[1], nstep, nsr, nrx, F.pad(er, _HALO),
+                         F.pad(se, _HALO), F.pad(mr, _HALO), 2 if nz == 1 else 3, dtype, pmlthick,
+                         source_amplitudes)
 
 
 def extended_shape_of(shape: Sequence[int], pml: Sequence[int]) -> Tuple[int, int, int]:

@@ -3,6 +3,7 @@ used for the print_parameters preview and for admission by HBM footprint."""
 from __future__ import annotations
 
 import math
+import os
 from typing import Dict
 
 from .pml import pml_phi_element_count
@@ -38,8 +39,9 @@ def estimate_compute_memory(nx: int, ny: int, nz: int, nt: int, nstep: int, pml,
         out["adjoint_fields"] = 6 * field
         out["adjoint_cpml"] = 2 * out["cpml"]
         out["gradients"] = 2 * cells * 4 + (2 * cells * 8 if fp64_accumulators else 0)
-        if nz > 1:
-            # 3D adjoint: six channel weights per cell for each half step
+        if nz > 1 and os.environ.get("DEEPGPR_AMD_3D_SWEEP") != "xmarch":
+            # the one-thread-per-cell 3D adjoint keeps six channel weights per
+            # cell for each half step; the x-marching sweeps form them on the fly
             out["adjoint_weights"] = 12 * nstep * (nx + 1) * (ny + 1) * (nz + 1) * 4
     out["total"] = sum(out.values())
     return out
