@@ -37,6 +37,16 @@ namespace {
 
 constexpr int kXmTZ = 64;        // tile width along z: one wavefront per tile row
 
+// Global-memory views of pointers read from the constant block: accesses
+// through them are global loads and stores. Accessed through a plain pointer
+// loaded from memory they would be flat accesses, which also count as LDS
+// traffic, so every LDS wait in the plane loop would wait for the next
+// plane's prefetch as well.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const float gcfloat;
+__device__ __forceinline__ gcfloat* gconst(const float* p) { return (gcfloat*)p; }
+__device__ __forceinline__ gfloat* gmut(float* p) { return (gfloat*)p; }
+
 // Logical tile of this workgroup (XCD-contiguous order when remapped: block b
 // runs on XCD b % 8, and XCD x takes tiles [x * per, (x + 1) * per)).
 __device__ __forceinline__ bool xm_tile(const XmTiles& xt, int& tile, int& y0, int& z0)
@@ -268,7 +278,7 @@ __device__ __forceinline__ float chan_weight_cpml(const XmConst* __restrict__ kc
 {
     const Grid& g = kc->g;
     const Cpml& cp = ELECTRIC ? kc->cpe : kc->cph;
-    const float* __restrict__ upd = ELECTRIC ? kc->cb : kc->hrhs;
+    gcfloat* upd = gconst(ELECTRIC ? kc->cb : kc->hrhs);
     const int c[3] = {c0, c1, c2};
     const int w = face_source(a, 0) == u ? 0 : 1;
     const int ca_ = a == 0 ? c0 : (a == 1 ? c1 : c2);
@@ -281,13 +291,13 @@ __device__ __forceinline__ float chan_weight_cpml(const XmConst* __restrict__ kc
         if (!psi_active(g, ELECTRIC, f, w, c)) continue;
         int dims[3];
         psi_dims(g, ELECTRIC, f, w, dims);
-        const float* Rc = cp.coef[f];
+        gcfloat* Rc = gconst(cp.coef[f]);
         const int pp = g.pml[f];
         const float ra1 = Rc[qq] - 1.0f, rf = Rc[3 * pp + qq];
         const float sign = ELECTRIC ? e_face_sign(a, w) : h_face_sign(a, w);
         const float uc = upd[m];
         const float inv = 1.0f / g.d[a];
-        const float old = cp.psi[f][w][psi_index(dims, s, c, a, qq)];
+        const float old = gconst(cp.psi[f][w])[psi_index(dims, s, c, a, qq)];
         acc += (sign * uc * ra1 * lt - rf * old) * inv;
     }
     return acc;
@@ -319,14 +329,14 @@ __device__ __forceinline__ void advance_lambda_psi(const XmConst* __restrict__ k
 {
     const Grid& g = kc->g;
     const Cpml& cp = ELECTRIC ? kc->cpe : kc->cph;
-    const float uc = (ELECTRIC ? kc->cb : kc->hrhs)[m];
+    const float uc = gconst(ELECTRIC ? kc->cb : kc->hrhs)[m];
     const int c[3] = {c0, c1, c2};
 #pragma unroll 1
     for (int f = 0; f < 6; ++f) {
         const int axis = f >> 1;
         const int qq = face_q(g, ELECTRIC, f, axis == 0 ? c0 : (axis == 1 ? c1 : c2));
         if (qq < 0) continue;
-        const float* Rc = cp.coef[f];
+        gcfloat* Rc = gconst(cp.coef[f]);
         const int pp = g.pml[f];
         const float rb = Rc[pp + qq], re = Rc[2 * pp + qq];
 #pragma unroll 1
@@ -335,7 +345,7 @@ __device__ __forceinline__ void advance_lambda_psi(const XmConst* __restrict__ k
             int dims[3];
             psi_dims(g, ELECTRIC, f, w, dims);
             const float sign = ELECTRIC ? e_face_sign(axis, w) : h_face_sign(axis, w);
-            float* psi = cp.psi[f][w];
+            gfloat* psi = gmut(cp.psi[f][w]);
             const long long pi = psi_index(dims, s, c, axis, qq);
             const int tg = face_target(axis, w);
             const float lt = tg == 0 ? l0 : (tg == 1 ? l1 : l2);
@@ -361,7 +371,7 @@ __device__ __forceinline__ Vec3 cpml_forward(const XmConst* __restrict__ kc, flo
 {
     const Grid& g = kc->g;
     const Cpml& cp = ELECTRIC ? kc->cpe : kc->cph;
-    const float upd = (ELECTRIC ? kc->cb : kc->hrhs)[m];
+    const float upd = gconst(ELECTRIC ? kc->cb : kc->hrhs)[m];
     const int c[3] = {c0, c1, c2};
     const float dv[6] = {d0, d1, d2, d3, d4, d5};
     float f[3] = {f0, f1, f2};
@@ -370,7 +380,7 @@ __device__ __forceinline__ Vec3 cpml_forward(const XmConst* __restrict__ kc, flo
         const int axis = fc >> 1;
         const int qq = face_q(g, ELECTRIC, fc, axis == 0 ? c0 : (axis == 1 ? c1 : c2));
         if (qq < 0) continue;
-        const float* Rc = cp.coef[fc];
+        gcfloat* Rc = gconst(cp.coef[fc]);
         const int pp = g.pml[fc];
         const float ra1 = Rc[qq] - 1.0f, rb = Rc[pp + qq], re = Rc[2 * pp + qq], rf = Rc[3 * pp + qq];
         const float da = g.d[axis];
@@ -381,7 +391,7 @@ __device__ __forceinline__ Vec3 cpml_forward(const XmConst* __restrict__ kc, flo
             psi_dims(g, ELECTRIC, fc, w, dims);
             const int tg = face_target(axis, w), sc = face_source(axis, w);
             const float der = pick6(dv, diff_slot(sc, axis)) / da;
-            float* psi = cp.psi[fc][w];
+            gfloat* psi = gmut(cp.psi[fc][w]);
             const long long pi = psi_index(dims, s, c, axis, qq);
             const float phi = psi[pi];
             const float corr = upd * (ra1 * der + rb * phi);
@@ -451,7 +461,7 @@ __global__ __launch_bounds__(64 * TY) void k_xm_fwd_h(Grid g, XmTiles xt, const 
     const long long sb = (long long)s * g.cells;
     const Cell cl(g, y0, z0);
     const int j = cl.j, k = cl.k, l = cl.l;
-    const float* __restrict__ hcurl = kc->hcurl;
+    gcfloat* hcurl = gconst(kc->hcurl);
     const bool has_cpml = g.pml[0] + g.pml[1] + g.pml[2] + g.pml[3] + g.pml[4] + g.pml[5] > 0;
 
     float q1[QN], q2[QN];  // Ey, Ez of the own column
@@ -597,9 +607,9 @@ __global__ __launch_bounds__(64 * TY) void k_xm_fwd_e(Grid g, XmTiles xt, const 
     const long long sb = (long long)s * g.cells;
     const Cell cl(g, y0, z0);
     const int j = cl.j, k = cl.k, l = cl.l;
-    const float* __restrict__ ca = kc->ca;
-    const float* __restrict__ ccurl = kc->ccurl;
-    const float* __restrict__ cb = kc->cb;
+    gcfloat* ca = gconst(kc->ca);
+    gcfloat* ccurl = gconst(kc->ccurl);
+    gcfloat* cb = gconst(kc->cb);
     const bool has_cpml = g.pml[0] + g.pml[1] + g.pml[2] + g.pml[3] + g.pml[4] + g.pml[5] > 0;
     // this workgroup's source entries, plane by plane
     int sp = 0, se = 0;
@@ -798,7 +808,8 @@ __global__ __launch_bounds__(64 * TY) void k_xm_adj(Grid g, XmTiles xt, const Xm
     const Cell cl(g, y0, z0);
     const int j = cl.j, k = cl.k, l = cl.l;
     // lw: the lambda the weights are formed from; lt: the lambda this sweep advances
-    const float* __restrict__ cw = ELECTRIC ? kc->ccurl : kc->hcurl;
+    gcfloat* cw = gconst(ELECTRIC ? kc->ccurl : kc->hcurl);
+    gcfloat* cav_src = gconst(kc->ca);
     const bool has_cpml = g.pml[0] + g.pml[1] + g.pml[2] + g.pml[3] + g.pml[4] + g.pml[5] > 0;
 
     // W of the x channels of the own cell at plane x: (1,0) from lambda_2, (2,0) from lambda_1
@@ -854,7 +865,7 @@ __global__ __launch_bounds__(64 * TY) void k_xm_adj(Grid g, XmTiles xt, const Xm
         nt0 = cl.ok ? lt.c[0][pb + cl.po] : 0.0f;
         nt1 = cl.ok ? lt.c[1][pb + cl.po] : 0.0f;
         nt2 = cl.ok ? lt.c[2][pb + cl.po] : 0.0f;
-        if constexpr (!ELECTRIC) nca = cl.ok ? kc->ca[mb + cl.po] : 0.0f;
+        if constexpr (!ELECTRIC) nca = cl.ok ? cav_src[mb + cl.po] : 0.0f;
         // y halo: W(0,1) from lambda_2, W(2,1) from lambda_0; z halo: W(0,2) from lambda_1, W(1,2) from lambda_0
         nha = cl.hv ? (cl.hy ? lw.c[2][pb + cl.hp] : lw.c[1][pb + cl.hp]) : 0.0f;
         nhb = cl.hv ? lw.c[0][pb + cl.hp] : 0.0f;
@@ -868,11 +879,8 @@ __global__ __launch_bounds__(64 * TY) void k_xm_adj(Grid g, XmTiles xt, const Xm
         float cav = 0.0f;
         if constexpr (!ELECTRIC) cav = nca;
         if (i + 1 < xb) load_plane(i + 1);
+        // staged channels: 0 W(0,1), 1 W(2,1) (y halo); 2 W(0,2), 3 W(1,2) (z halo)
         const int bsel = (i - xa) & 1;
-        float* y0w = sm[bsel][0];  // W(0,1)
-        float* y2w = sm[bsel][1];  // W(2,1)
-        float* z0w = sm[bsel][2];  // W(0,2)
-        float* z1w = sm[bsel][3];  // W(1,2)
         const long long mb = (long long)i * g.st0;
         // The plane's weights, one item at a time: 0-1 the x channels of the
         // own column at the lead plane (through sx into the queue), 2-5 the y
@@ -883,27 +891,29 @@ __global__ __launch_bounds__(64 * TY) void k_xm_adj(Grid g, XmTiles xt, const Xm
             const bool lead = it < 2, halo = it >= 6;
             if (halo && cl.hl < 0) continue;
             const bool valid = lead ? (cl.ok && i + LEAD < NX) : (halo ? cl.hv : cl.ok);
-            int u, a;
+            int u, a, ch, idx;
             float lam, cu;
-            float* dst;
             if (lead) {
                 u = it + 1;
                 a = 0;
                 lam = it == 0 ? ll2 : ll1;
                 cu = llc;
-                dst = &sx[it][tid];
+                ch = -1;
+                idx = it;
             } else if (!halo) {
                 u = it == 3 ? 2 : (it == 5 ? 1 : 0);
                 a = it < 4 ? 1 : 2;
                 lam = it == 2 ? w2 : (it == 4 ? w1 : w0);
                 cu = wc;
-                dst = (it == 2 ? y0w : (it == 3 ? y2w : (it == 4 ? z0w : z1w))) + l;
+                ch = it - 2;
+                idx = l;
             } else {
                 a = cl.hy ? 1 : 2;
                 u = it == 6 ? 0 : (cl.hy ? 2 : 1);
                 lam = it == 6 ? ha : hb;
                 cu = hc;
-                dst = (it == 6 ? (cl.hy ? y0w : z0w) : (cl.hy ? y2w : z1w)) + cl.hl;
+                ch = (cl.hy ? 0 : 2) + (it - 6);
+                idx = cl.hl;
             }
             float wv = 0.0f;
             if (valid) {
@@ -912,7 +922,8 @@ __global__ __launch_bounds__(64 * TY) void k_xm_adj(Grid g, XmTiles xt, const Xm
                 const bool cpml = has_cpml && in_any_face(g, ELECTRIC, c);
                 wv = chan_weight<ELECTRIC>(kc, g, u, a, lam, cu, cpml, c, m, s);
             }
-            *dst = wv;
+            if (lead) sx[idx][tid] = wv;
+            else sm[bsel][ch][idx] = wv;
         }
         q1[QN - 1] = sx[0][tid];
         q2[QN - 1] = sx[1][tid];
@@ -930,13 +941,13 @@ __global__ __launch_bounds__(64 * TY) void k_xm_adj(Grid g, XmTiles xt, const Xm
                     float acc = tv[u];
                     if (u != 0) acc += u == 1 ? qgather_bwd_T<ORDER, B>(q1, i, NX) : qgather_bwd_T<ORDER, B>(q2, i, NX);
                     if (u != 1) {
-                        const float* W = u == 0 ? y0w : y2w;
-                        auto wf = [&](int off) { return W[l + off * RZ]; };
+                        const int ch = u == 0 ? 0 : 1;
+                        auto wf = [&](int off) { return sm[bsel][ch][l + off * RZ]; };
                         acc += gather_bwd_T<ORDER>(wf, j, NY);
                     }
                     if (u != 2) {
-                        const float* W = u == 0 ? z0w : z1w;
-                        auto wf = [&](int off) { return W[l + off]; };
+                        const int ch = u == 0 ? 2 : 3;
+                        auto wf = [&](int off) { return sm[bsel][ch][l + off]; };
                         acc += gather_bwd_T<ORDER>(wf, k, NZ);
                     }
                     lt.c[u][n] = acc;
@@ -949,13 +960,13 @@ __global__ __launch_bounds__(64 * TY) void k_xm_adj(Grid g, XmTiles xt, const Xm
                     if (e_updated(g, t, c)) acc = cav * acc;
                     if (t != 0) acc += t == 1 ? qgather_fwd_T<ORDER, B>(q1, i, NX) : qgather_fwd_T<ORDER, B>(q2, i, NX);
                     if (t != 1) {
-                        const float* W = t == 0 ? y0w : y2w;
-                        auto wf = [&](int off) { return W[l + off * RZ]; };
+                        const int ch = t == 0 ? 0 : 1;
+                        auto wf = [&](int off) { return sm[bsel][ch][l + off * RZ]; };
                         acc += gather_fwd_T<ORDER>(wf, j, NY);
                     }
                     if (t != 2) {
-                        const float* W = t == 0 ? z0w : z1w;
-                        auto wf = [&](int off) { return W[l + off]; };
+                        const int ch = t == 0 ? 2 : 3;
+                        auto wf = [&](int off) { return sm[bsel][ch][l + off]; };
                         acc += gather_fwd_T<ORDER>(wf, k, NZ);
                     }
                     lt.c[t][n] = acc;
