@@ -4,52 +4,21 @@ The native calls made by the autograd layer are captured (inputs before,
 outputs after) and replayed through the CPU oracle, so the whole host path -
 model extension, CPML profiles, location shift, history allocation and the
 storage_type bit field - is checked bit for bit against the oracle."""
+import os
+import sys
+
 import numpy as np
 import pytest
 import torch
 
+sys.path.insert(0, os.path.dirname(__file__))
+
 pytestmark = pytest.mark.gpu
 
 
-def _capture(monkeypatch):
-    import deepgpr_amd.solver.autograd as ag
-    calls = []
-    real = ag.invoke
-
-    def spy(function, args, device):
-        snap = {k: ([t.detach().clone() if isinstance(t, torch.Tensor) else t for t in v]
-                    if isinstance(v, (list, tuple)) else (v.detach().clone() if isinstance(v, torch.Tensor) else v))
-                for k, v in args.items()}
-        real(function, args, device)
-        calls.append((function, snap, args))
-
-    monkeypatch.setattr(ag, "invoke", spy)
-    return calls
-
-
-def _np(t):
-    return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else t
-
-
-def _problem(h, fwd_args, bwd_args=None):
-    a = fwd_args
-    p = h.Problem(a["nx_fields"], a["ny_fields"], a["nz_fields"], a["nshot"], a["nt"], a["fdtd_order"],
-                  list(a["pml"]), (a["dx"], a["dy"], a["dz"]), a["dt"], _np(a["eps_r_pad"]), _np(a["sigma_pad"]),
-                  _np(a["mu_r_pad"]), [_np(t) for t in a["pml_coeff"][:6]], [_np(t) for t in a["pml_coeff"][6:]],
-                  [_np(t) for t in a["E"]], [_np(t) for t in a["H"]], [_np(t) for t in a["phi"]],
-                  _np(a["source_location"]), _np(a["source_waveform"]).reshape(a["nsource"], a["nt"]),
-                  _np(a["receiver_location"]))
-    p.src_comp, p.rx_comp = a["source_component"], a["receiver_component"]
-    p.S, p.mode = a["sampling_interval"], a["fwi_mode"]
-    p.storage_type = a["storage_type"] & ~(1 << 6)
-    p.save_model_history = a["save_model_history"]
-    if bwd_args is not None:
-        b = bwd_args
-        p.lam0 = [_np(t) for t in b["E"]] + [_np(t) for t in b["H"]] + [_np(t) for t in b["phi"]]
-        p.data_grad = _np(b["data_grad"])
-        p.want_src, p.want_eps, p.want_sig = (b["source_requires_grad"], b["eps_r_requires_grad"],
-                                              b["sigma_requires_grad"])
-    return p
+from gpu_helpers import capture as _capture  # noqa: E402
+from gpu_helpers import np_of as _np  # noqa: E402
+from gpu_helpers import problem as _problem  # noqa: E402
 
 
 def _c1_inputs(dev):
