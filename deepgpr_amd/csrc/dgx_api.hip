@@ -29,6 +29,38 @@ void set_error(const char* fmt, ...)
     va_end(ap);
 }
 
+// Scratch of dgx_forward_noalloc: a file-scope device array (each GPU the
+// library runs on holds its own copy until the process ends). Host code uses
+// the address hipGetSymbolAddress returns for the current device, never the
+// array's name. Calls that overlap on one GPU share it and race.
+constexpr size_t kNoAllocScratchBytes = 4u << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (looked up once per device).
+void* noalloc_scratch()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("dgx_forward_noalloc: no usable current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !ptr) {
+            set_error("dgx_forward_noalloc: hipGetSymbolAddress of the scratch array failed: %s",
+                      hipGetErrorString(rc));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 // Device allocation released at scope exit (after the stream has drained).
 // Stream of the solve running on this thread. Scratch is allocated and freed
 // stream-ordered on it, so concurrent solves on different streams never wait
@@ -773,7 +805,10 @@ bool upload(DevBuf& d, const std::vector<T>& v, hipStream_t st)
            hipMemcpyAsync(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) == hipSuccess;
 }
 
-int run_forward(const dgx_solve* P, hipStream_t stream)
+// With `fixed` (dgx_forward_noalloc), the solve allocates no device memory:
+// it runs the one-thread-per-cell sweeps and places the electric sweep's
+// source lists in `fixed` (fixed_bytes long) instead of allocating them.
+int run_forward(const dgx_solve* P, hipStream_t stream, void* fixed = nullptr, size_t fixed_bytes = 0)
 {
     CallStream call(stream);
     Plan pl;
@@ -820,10 +855,14 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
         }
 
     const bool inject = P->nsource > 0 && P->source_component >= 0 && P->source_component <= 2;
-    const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
+    const bool tm = !fixed && tm2d_applies(P, pl, hsloc, hrloc);
     const bool fused = tm && tm2d_fused();
-    const bool xm = !tm && xmarch3d();
+    const bool xm = !fixed && !tm && xmarch3d();
     const XmTiles xt = xm_tiles(g, pl.order);
+    if (fixed && pl.kind == kInt8 && pl.history) {
+        set_error("dgx_forward_noalloc: INT8 histories need a scratch frame; use dgx_forward");
+        return -1;
+    }
     const Tm2dGrid tg = tm_grid_of(pl, tm2d_tile_rows(pl));
     const int tm_tiles = (g.N[0] * g.N[1] + 255) / 256;
     long long nblocks = tm ? (long long)tm_tiles * g.nshot
@@ -868,16 +907,32 @@ int run_forward(const dgx_solve* P, hipStream_t stream)
     DevBuf d_boff, d_ents;
     SourceBlocks src{};
     if (inject) {
-        if (!d_boff.alloc(boff.size() * sizeof(int)) || !d_ents.alloc(ents.size() * sizeof(int2))) {
-            set_error("forward: out of device memory for source lists");
-            return -1;
+        void* off_p = nullptr;
+        void* ent_p = nullptr;
+        const size_t off_bytes = boff.size() * sizeof(int), ent_bytes = ents.size() * sizeof(int2);
+        if (fixed) {
+            const size_t at = (off_bytes + 255) / 256 * 256;
+            if (at + ent_bytes > fixed_bytes) {
+                set_error("dgx_forward_noalloc: the source lists need %zu bytes, more than the %zu-byte scratch "
+                          "(fewer shots per call, or dgx_forward)", at + ent_bytes, fixed_bytes);
+                return -1;
+            }
+            off_p = fixed;
+            ent_p = (unsigned char*)fixed + at;
+        } else {
+            if (!d_boff.alloc(off_bytes) || !d_ents.alloc(ent_bytes)) {
+                set_error("forward: out of device memory for source lists");
+                return -1;
+            }
+            off_p = d_boff.p;
+            ent_p = d_ents.p;
         }
-        DGX_TRY(hipMemcpyAsync(d_boff.p, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice, stream),
+        DGX_TRY(hipMemcpyAsync(off_p, boff.data(), off_bytes, hipMemcpyHostToDevice, stream),
                 "forward: upload source lists");
-        DGX_TRY(hipMemcpyAsync(d_ents.p, ents.data(), ents.size() * sizeof(int2), hipMemcpyHostToDevice, stream),
+        DGX_TRY(hipMemcpyAsync(ent_p, ents.data(), ent_bytes, hipMemcpyHostToDevice, stream),
                 "forward: upload source entries");
-        src.block_off = (const int*)d_boff.p;
-        src.entries = (const int2*)d_ents.p;
+        src.block_off = (const int*)off_p;
+        src.entries = (const int2*)ent_p;
         src.count = (int)ents.size();
         src.wave = P->source_waveform;
         src.nt = nt;
@@ -1893,6 +1948,16 @@ DGX_API int dgx_forward(const dgx_solve* solve, dgx_stream_t stream)
     g_error[0] = '\0';
     return run_forward(solve, (hipStream_t)stream);
 }
+
+DGX_API int dgx_forward_noalloc(const dgx_solve* solve, dgx_stream_t stream)
+{
+    g_error[0] = '\0';
+    void* fixed = noalloc_scratch();
+    if (!fixed) return -1;
+    return run_forward(solve, (hipStream_t)stream, fixed, kNoAllocScratchBytes);
+}
+
+DGX_API long long dgx_forward_noalloc_scratch_bytes(void) { return (long long)kNoAllocScratchBytes; }
 
 DGX_API int dgx_backward(const dgx_solve* solve, dgx_stream_t stream)
 {

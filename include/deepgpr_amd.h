@@ -110,6 +110,19 @@ DGX_API int dgx_forward(const dgx_solve* solve, dgx_stream_t stream);
 /* Exact discrete adjoint; replaces backward() of deepgpr.h:117-152.
  * Stream-ordered like dgx_forward. */
 DGX_API int dgx_backward(const dgx_solve* solve, dgx_stream_t stream);
+/* No counterpart in the reference. dgx_forward for callers that cannot
+ * allocate device memory (serving loops that must not allocate per request,
+ * harnesses that forbid hipMalloc): it calls no hipMalloc, hipFree or other
+ * allocating routine. It runs the one-thread-per-cell sweeps (results are
+ * those of dgx_forward, bit for bit) and keeps its only scratch, the source
+ * lists of the electric sweep, in a fixed device array of
+ * dgx_forward_noalloc_scratch_bytes() bytes (4 MiB) per GPU: a solve whose
+ * lists need more (about 4 bytes per 256-cell workgroup of the sweep over all
+ * shots, plus 8 bytes per source) is rejected with an error, and so is an
+ * INT8 history (it needs a scratch frame). Calls that overlap on one GPU
+ * share the array and race. */
+DGX_API int dgx_forward_noalloc(const dgx_solve* solve, dgx_stream_t stream);
+DGX_API long long dgx_forward_noalloc_scratch_bytes(void);
 /* FP16/BF16 history codec round trip on device arrays; replaces
  * deepgpr_test_wavefield_conversion (deepgpr.h:84-86). */
 DGX_API int dgx_codec_roundtrip(const float* input, void* encoded, float* decoded, long long count,

@@ -399,7 +399,9 @@ def run_cpu(p: Problem, which: str = "oracle", real=np.float32, backward: bool =
 
 
 def run_gpu(p: Problem, backward: bool = True, api: str = "abi6", fp64: bool = False, device: str = "cuda:0") -> Dict:
-    """The same problem through the HIP library (ABI-6 exports or dgx_*)."""
+    """The same problem through the HIP library: api 'abi6' (the ABI-6
+    exports), 'dgx' (dgx_forward/dgx_backward) or 'dgx_noalloc'
+    (dgx_forward_noalloc, then dgx_backward)."""
     import torch
 
     from deepgpr_amd.native.bridge import invoke
@@ -445,7 +447,7 @@ def run_gpu(p: Problem, backward: bool = True, api: str = "abi6", fp64: bool = F
     else:
         args = dict(common, E=st["E"], H=st["H"], update_coeff=coeffs, phi=st["phi"], receiver_data=rdata,
                     source_waveform=st["wave"], save_model_history=p.save_model_history)
-        invoke("forward", args, dev)
+        invoke("forward_noalloc" if api == "dgx_noalloc" else "forward", args, dev)
     torch.cuda.synchronize(dev)
     N = lambda t: t.cpu().numpy()  # noqa: E731
     out = {"E": [N(x) for x in st["E"]], "H": [N(x) for x in st["H"]], "phi": [N(x) for x in st["phi"]],
