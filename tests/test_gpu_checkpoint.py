@@ -1,0 +1,85 @@
+"""Segment checkpointing on the GPU (restates the reference's
+tests/test_external_pml.py:145-191 for this build): a solve split into
+segments under torch.utils.checkpoint, each segment starting from the states
+the previous one returned (checkpoint_initial_field for the first), gives the
+traces and final states of one unsegmented solve bit for bit, and the eps_r,
+sigma and source-waveform gradients of the unsegmented adjoint within float32
+summation round-off, through two FWI updates, with re-entrant and
+non-re-entrant checkpoints, in 2D and 3D.
+
+The gradients cannot be the same bits: the unsegmented adjoint sums every
+time step's terms into one accumulator in reverse time order, while autograd
+adds the per-segment sums (each summed from zero), which associates the same
+terms differently."""
+import pytest
+import torch
+from torch.utils.checkpoint import checkpoint
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(shape, pml, device):
+    n = 1
+    for v in shape:
+        n *= v
+    model = torch.linspace(3.0, 5.0, steps=n, device=device).reshape(shape)
+    sigma = torch.full_like(model, 2.0e-4)
+    mu = torch.linspace(1.0, 1.2, steps=n, device=device).reshape(shape)
+    nt = 35
+    source = torch.zeros((1, nt, 1), device=device)
+    source[0, 1:4, 0] = torch.tensor([1.0, -0.5, 0.25], device=device)
+    z = 0 if len(shape) == 2 else 2
+    args = dict(device=device, dx=(0.02, 0.017, 0.015), dt=1.0e-11, source_amplitudes=source,
+                source_location=torch.tensor([[[0, 3, z]], [[3, 4, z]]], device=device),
+                receiver_location=torch.tensor([[[0, 3, z], [3, 4, z]], [[3, 4, z], [2, 3, z]]], device=device),
+                pmlthick=pml, mu_r=mu, mode=2 if z == 0 else 3, gradient_accumulator="float32")
+    return model, sigma, args
+
+
+@pytest.mark.parametrize("shape,pml,reentrant", [
+    ((8, 9), (2, 3, 1, 2), True), ((6, 7, 5), (1, 2, 2, 1, 1, 2), True),
+    ((8, 9), (2, 3, 1, 2), False), ((6, 7, 5), (1, 2, 2, 1, 1, 2), False),
+    ((30, 26, 34), (4, 5, 3, 4, 5, 3), False),
+])
+def test_checkpoint_segments_match_the_full_solve(cuda_device, shape, pml, reentrant):
+    import deepgpr_amd as dg
+    print(f"[gpu checkpoint] {shape} reentrant={reentrant}", flush=True)
+    base, sigma_base, args = _problem(shape, pml, cuda_device)
+    model = base.clone().requires_grad_()
+    sigma = sigma_base.clone().requires_grad_()
+    opt = torch.optim.SGD([model, sigma], lr=1.0e-7)
+    seg_len = 12
+    for _ in range(2):
+        ref_model = model.detach().clone().requires_grad_()
+        ref_sigma = sigma.detach().clone().requires_grad_()
+        ref_source = args["source_amplitudes"].clone().requires_grad_()
+        full = dg.compute(eps_r=ref_model, sigma=ref_sigma, **{**args, "source_amplitudes": ref_source})
+        full[-1].square().sum().backward()
+
+        source = args["source_amplitudes"].clone().requires_grad_()
+        init = {k: v for k, v in args.items() if k not in ("mode", "mu_r", "gradient_accumulator")}
+        E, H, P = dg.checkpoint_initial_field(er=model, se=sigma, mr=args["mu_r"], **init)
+        state = (*E, *H, *P)
+        traces = []
+        for start in range(0, source.shape[1], seg_len):
+            def segment(er, se, wave, *boundary):
+                work = tuple(t.clone() for t in boundary)
+                r = dg.compute(eps_r=er, sigma=se, **{**args, "source_amplitudes": wave}, E=work[:3], H=work[3:6],
+                               PML=work[6:])
+                return (*r[1], *r[2], *r[3], r[-1])
+            out = checkpoint(segment, model, sigma, source[:, start:start + seg_len], *state, use_reentrant=reentrant)
+            state, data = out[:-1], out[-1]
+            traces.append(data)
+        data = torch.cat(traces, dim=1)
+        assert torch.equal(data, full[-1])
+        for a, b in zip(state, (*full[1], *full[2], *full[3])):
+            assert torch.equal(a, b)
+        opt.zero_grad()
+        data.square().sum().backward()
+        for a, b in ((model.grad, ref_model.grad), (sigma.grad, ref_sigma.grad), (source.grad, ref_source.grad)):
+            torch.testing.assert_close(a, b, rtol=2.0e-5, atol=max(1.0e-5, 1.0e-7 * float(b.abs().max())))
+        opt.step()
+        with torch.no_grad():
+            model.clamp_(min=1.0)
+            sigma.clamp_(min=0.0)
+    assert not torch.equal(model, base)
