@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dgx_sweep.h"
+
 namespace dgx {
 
 constexpr float kEps0 = 8.8541878128e-12f;   // deepgpr.h:23
@@ -163,7 +165,50 @@ __device__ __forceinline__ float dfwd_full(const float* __restrict__ f, IDX id, 
     }
 }
 
-// Transposes of the staggered differences (deepgpr_cpu.c:430-511).
+// Transposes of the staggered differences (deepgpr_cpu.c:430-511). The
+// _full forms are the branches taken at least 2R - 1 cells from both faces
+// (c >= 2R - 1 and c <= n - 2R), where no stencil is clamped.
+template <int ORDER, typename WF>
+__device__ __forceinline__ float gather_bwd_full(WF wf)
+{
+    if constexpr (ORDER <= 2) {
+        float acc = wf(0);
+        acc -= wf(1);
+        return acc;
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+            const float kc = stencil_c(R, r);
+            acc += kc * wf(-(r - 1));
+            acc -= kc * wf(r);
+        }
+        return acc;
+    }
+}
+
+template <int ORDER, typename WF>
+__device__ __forceinline__ float gather_fwd_full(WF wf)
+{
+    if constexpr (ORDER <= 2) {
+        float acc = 0.0f;
+        acc += wf(-1);
+        acc -= wf(0);
+        return acc;
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+            const float kc = stencil_c(R, r);
+            acc += kc * wf(-r);
+            acc -= kc * wf(r - 1);
+        }
+        return acc;
+    }
+}
+
 template <int ORDER, typename WF>
 __device__ __forceinline__ float gather_bwd_T(WF wf, int c, int n)
 {
@@ -174,15 +219,7 @@ __device__ __forceinline__ float gather_bwd_T(WF wf, int c, int n)
     } else {
         constexpr int R = Radius<ORDER>::value;
         float acc = 0.0f;
-        if (c >= 2 * R - 1 && c + 2 * R - 1 < n) {
-#pragma unroll
-            for (int r = 1; r <= R; ++r) {
-                const float kc = stencil_c(R, r);
-                acc += kc * wf(-(r - 1));
-                acc -= kc * wf(r);
-            }
-            return acc;
-        }
+        if (c >= 2 * R - 1 && c + 2 * R - 1 < n) return gather_bwd_full<ORDER>(wf);
         for (int r = 1; r <= R; ++r) {
             const int sp = c - (r - 1);
             if (sp >= 0) {
@@ -210,15 +247,7 @@ __device__ __forceinline__ float gather_fwd_T(WF wf, int c, int n)
     } else {
         constexpr int R = Radius<ORDER>::value;
         float acc = 0.0f;
-        if (c >= 2 * R - 1 && c + 2 * R <= n) {
-#pragma unroll
-            for (int r = 1; r <= R; ++r) {
-                const float kc = stencil_c(R, r);
-                acc += kc * wf(-r);
-                acc -= kc * wf(r - 1);
-            }
-            return acc;
-        }
+        if (c >= 2 * R - 1 && c + 2 * R <= n) return gather_fwd_full<ORDER>(wf);
         for (int r = 1; r <= R; ++r) {
             const int sp = c - r;
             if (sp >= 0) {
@@ -322,33 +351,6 @@ constexpr int kDiffAxis[6] = {1, 2, 2, 0, 0, 1};
 __device__ __forceinline__ int diff_slot(int comp, int axis)
 {
     return axis == 0 ? (comp == 2 ? 3 : 4) : (axis == 1 ? (comp == 2 ? 0 : 5) : (comp == 1 ? 1 : 2));
-}
-
-// Cells of a cell sweep: every row j of the plane, and in each row the k of
-// two runs [za, za+na) and [zb, zb+nb). A sweep is one launch of up to two
-// such parts: workgroups with blockIdx.x < split take part 0 (the z-interior
-// of every row), the others part 1 (the z-CPML cells), so a wavefront never
-// mixes the two while both parts share one launch.
-struct ZRun {
-    int za, na, zb, nb;
-};
-struct ZRuns {
-    ZRun run[2];
-    int split;
-};
-
-__device__ __forceinline__ bool run_cell(const Grid& g, const ZRuns& zr, int& j, int& k)
-{
-    const bool second = (int)blockIdx.x >= zr.split;
-    const ZRun z = second ? zr.run[1] : zr.run[0];
-    const int bx = (int)blockIdx.x - (second ? zr.split : 0);
-    const int per = z.na + z.nb;
-    const int q = bx * (int)blockDim.x + (int)threadIdx.x;
-    if (q >= per * g.N[1]) return false;
-    j = q / per;
-    const int r = q - j * per;
-    k = r < z.na ? z.za + r : z.zb + (r - z.na);
-    return true;
 }
 
 // Profile index of coordinate c on a face, or -1 outside the face region.

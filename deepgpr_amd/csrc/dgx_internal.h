@@ -258,13 +258,14 @@ void launch_tm_adj_h(int order, const Tm2dGrid& g, const Tm2dFields& f, const Tm
 void launch_tm_material_gradient(const Tm2dGrid& g, const Tm2dFields& f, const MatGrad& mg, int ez_comp,
                                  void* geps, void* gsig, bool fp64, hipStream_t st);
 
-// The launches of a one-thread-per-cell sweep (see cell_sweep).
+// The launches of a one-thread-per-cell sweep (block -> cell map in dgx_sweep.h).
 struct CellSweep {
-    ZRuns zr;
-    dim3 grid;
+    CellMap cm;
+    dim3 grid;         // (blocks per shot, shots)
     long long blocks;  // workgroups of the launch
 };
-CellSweep cell_sweep(const Grid& g, int order, bool split = true);
+// use_box: advance the interior box with its own plain-cell path.
+CellSweep cell_sweep(const Grid& g, int order, bool use_box = true);
 // Workgroup (linear index in the launch) that advances cell (i, j, k) of shot s.
 long long cell_block(const CellSweep& cs, const Grid& g, int i, int j, int k, int s);
 void launch_coefficients(const Grid& g, const float* eps, const float* sig, const float* mu, float* const co[6],

@@ -6,6 +6,8 @@
 // re-evaluates the transposed stencil weights of its neighbours, so no
 // atomics are used anywhere and results do not depend on scheduling.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "dgx_device.h"
 #include "dgx_internal.h"
@@ -48,43 +50,50 @@ __global__ __launch_bounds__(256) void k_coefficients(
 // ---------------------------------------------------------------------------
 // Forward magnetic half step: curl update + H-CPML (deepgpr_cpu.c:886-1100)
 // ---------------------------------------------------------------------------
+// A plain cell (every component updated, no clamped difference, no CPML
+// term): the curl update alone. m: cell in the model, n: cell of the shot.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_fwd_h(Grid g, const float* __restrict__ hcurl,
-                                               const float* __restrict__ hrhs, CF3 E, F3 H, Cpml cp,
-                                               int has_cpml, ZRuns zr)
+__device__ __forceinline__ void fwd_h_plain(const Grid& g, const float* __restrict__ hcurl, const CF3& E,
+                                            const F3& H, long long m, long long n)
 {
-    int j, k;
-    if (!run_cell(g, zr, j, k)) return;
-    const int p = j * g.N[2] + k;
-    const int i = blockIdx.y, s = blockIdx.z;
+    const long long sx = g.st0, sy = g.N[2];
+    float h0 = H.c[0][n], h1 = H.c[1][n], h2 = H.c[2][n];
+    const float u1 = hcurl[m];
+    const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
+    {
+        const float dEz = dfwd_full<ORDER>(E.c[2], n, sy);
+        const float dEy = dfwd_full<ORDER>(E.c[1], n, 1);
+        h0 = h0 - uy * dEz + uz * dEy;
+    }
+    {
+        const float dEx = dfwd_full<ORDER>(E.c[0], n, 1);
+        const float dEz = dfwd_full<ORDER>(E.c[2], n, sx);
+        h1 = h1 - uz * dEx + u1 * dEz;
+    }
+    {
+        const float dEy = dfwd_full<ORDER>(E.c[1], n, sx);
+        const float dEx = dfwd_full<ORDER>(E.c[0], n, sy);
+        h2 = h2 - u1 * dEy + uy * dEx;
+    }
+    H.c[0][n] = h0;
+    H.c[1][n] = h1;
+    H.c[2][n] = h2;
+}
+
+// A shell cell of the magnetic half step: CPML terms, clamped differences
+// and update masks as the cell needs them.
+template <int ORDER>
+__device__ __forceinline__ void fwd_h_cell(const Grid& g, const float* __restrict__ hcurl,
+                                           const float* __restrict__ hrhs, const CF3& E, const F3& H, const Cpml& cp,
+                                           int has_cpml, int i, int j, int k, int s)
+{
     const int c[3] = {i, j, k};
-    const long long m = (long long)i * g.st0 + p;
+    const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
     const long long n = (long long)s * g.cells + m;
     const long long st[3] = {g.st0, (long long)g.N[2], 1};
     const bool pml = has_cpml && in_any_face(g, false, c);
     if (!pml && plain_cell(g, Radius<ORDER>::value, c)) {
-        // every component updated, no clamped difference, no CPML term
-        float h0 = H.c[0][n], h1 = H.c[1][n], h2 = H.c[2][n];
-        const float u1 = hcurl[m];
-        const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
-        {
-            const float dEz = dfwd_full<ORDER>(E.c[2], n, st[1]);
-            const float dEy = dfwd_full<ORDER>(E.c[1], n, 1);
-            h0 = h0 - uy * dEz + uz * dEy;
-        }
-        {
-            const float dEx = dfwd_full<ORDER>(E.c[0], n, 1);
-            const float dEz = dfwd_full<ORDER>(E.c[2], n, st[0]);
-            h1 = h1 - uz * dEx + u1 * dEz;
-        }
-        {
-            const float dEy = dfwd_full<ORDER>(E.c[1], n, st[0]);
-            const float dEx = dfwd_full<ORDER>(E.c[0], n, st[1]);
-            h2 = h2 - u1 * dEy + uy * dEx;
-        }
-        H.c[0][n] = h0;
-        H.c[1][n] = h1;
-        H.c[2][n] = h2;
+        fwd_h_plain<ORDER>(g, hcurl, E, H, m, n);
         return;
     }
     const bool up0 = h_updated(g, 0, c), up1 = h_updated(g, 1, c), up2 = h_updated(g, 2, c);
@@ -146,36 +155,111 @@ __global__ __launch_bounds__(256) void k_fwd_h(Grid g, const float* __restrict__
     if (up2 || pml) H.c[2][n] = h[2];
 }
 
+template <int ORDER>
+__global__ __launch_bounds__(256, ORDER == 8 ? 5 : 8) void k_fwd_h(Grid g, const float* __restrict__ hcurl,
+                                               const float* __restrict__ hrhs, CF3 E, F3 H, Cpml cp,
+                                               int has_cpml, CellMap cm)
+{
+    const int s = blockIdx.y;
+    int i, j, k;
+    int idx;
+    const int part = sweep_part(cm, blockIdx.x, idx);
+    if (part < 0) return;
+    if (part == 1) {
+        // interior box: every cell plain
+        if (!box_cell(cm, idx, threadIdx.x, i, j, k)) return;
+        const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
+        fwd_h_plain<ORDER>(g, hcurl, E, H, m, (long long)s * g.cells + m);
+        return;
+    }
+    if (!shell_cell(cm, idx, threadIdx.x, i, j, k)) return;
+    fwd_h_cell<ORDER>(g, hcurl, hrhs, E, H, cp, has_cpml, i, j, k, s);
+}
+
 // ---------------------------------------------------------------------------
 // Forward electric half step: curl update + E-CPML + sources + history
 // (deepgpr_cpu.c:670-883, 644-668, 1559-1633)
 // ---------------------------------------------------------------------------
+// The curl update of a plain cell: e holds E^n and receives E^(n+1).
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__ ca,
-                                               const float* __restrict__ ccurl,
-                                               const float* __restrict__ cb, CF3 Hf, F3 E, Cpml cp,
-                                               int has_cpml, SourceBlocks src, HistWrite hist, ZRuns zr)
+__device__ __forceinline__ void fwd_e_plain(const Grid& g, const CF3& Hf, long long n, float u0, float u1,
+                                            float uy, float uz, float (&e)[3])
 {
-    int j, k;
-    if (!run_cell(g, zr, j, k)) return;
-    const int p = j * g.N[2] + k;
-    const int i = blockIdx.y, s = blockIdx.z;
+    const long long sx = g.st0, sy = g.N[2];
+    {
+        const float dHz = dbwd_full<ORDER>(Hf.c[2], n, sy);
+        const float dHy = dbwd_full<ORDER>(Hf.c[1], n, 1);
+        e[0] = u0 * e[0] + uy * dHz - uz * dHy;
+    }
+    {
+        const float dHx = dbwd_full<ORDER>(Hf.c[0], n, 1);
+        const float dHz = dbwd_full<ORDER>(Hf.c[2], n, sx);
+        e[1] = u0 * e[1] + uz * dHx - u1 * dHz;
+    }
+    {
+        const float dHy = dbwd_full<ORDER>(Hf.c[1], n, sx);
+        const float dHx = dbwd_full<ORDER>(Hf.c[0], n, sy);
+        e[2] = u0 * e[2] + u1 * dHy - uy * dHx;
+    }
+}
+
+// After the curl (and CPML) update of a cell: the Hertzian dipole sources of
+// its block in source order, the history (E^n before the update and the
+// effective right-hand side R^n), then the stores of the changed components.
+__device__ __forceinline__ void fwd_e_finish(const SourceBlocks& src, const HistWrite& hist, int sb, int se,
+                                             bool in_box, int i, int j, int k, int s, long long m, long long n,
+                                             float (&e)[3], const float (&eold)[3], const bool (&up)[3], float u0,
+                                             float cbv, bool touched, const F3& E)
+{
+    for (int q = sb; q < se; ++q) {
+        const int2 ent = src.entries[q];
+        if ((long long)ent.x == m) {
+            const float scale = src.wave[(long long)ent.y * src.nt + src.it] * src.dl / src.vol;
+            e[src.comp] -= cbv * scale;
+        }
+    }
+    if (in_box) {
+        const long long h = ((long long)(i - hist.box.x0) * hist.box.ny + (j - hist.box.y0)) * hist.box.nz +
+                            (k - hist.box.z0);
+        const long long base = (long long)s * hist.box.cells + h;
+        const int comps = hist.mode == 3 ? 3 : 1;
+        for (int ci = 0; ci < comps; ++ci) {
+            const int comp = hist.mode == 3 ? ci : 2;
+            if (hist.save_e) store_hist(hist.e_base, ci * hist.e_cstride + base, eold[comp], hist.e_kind);
+            if (hist.save_r) {
+                const float a = up[comp] ? u0 : 1.0f;
+                const float rhs = cbv != 0.0f ? (e[comp] - a * eold[comp]) / cbv : 0.0f;
+                store_hist(hist.r_base, ci * hist.r_cstride + base, rhs, hist.r_kind);
+            }
+        }
+    }
+    if (up[0] || touched) E.c[0][n] = e[0];
+    if (up[1] || touched) E.c[1][n] = e[1];
+    if (up[2] || touched) E.c[2][n] = e[2];
+}
+
+__device__ __forceinline__ bool in_hist_box(const HistWrite& hist, int i, int j, int k)
+{
+    return hist.active && i >= hist.box.x0 && i < hist.box.x0 + hist.box.nx && j >= hist.box.y0 &&
+           j < hist.box.y0 + hist.box.ny && k >= hist.box.z0 && k < hist.box.z0 + hist.box.nz;
+}
+
+// A shell cell of the electric half step; sb..se are its block's source
+// entries.
+template <int ORDER>
+__device__ __forceinline__ void fwd_e_cell(const Grid& g, const float* __restrict__ ca,
+                                           const float* __restrict__ ccurl, const float* __restrict__ cb,
+                                           const CF3& Hf, const F3& E, const Cpml& cp, int has_cpml,
+                                           const SourceBlocks& src, const HistWrite& hist, int sb, int se, int i,
+                                           int j, int k, int s)
+{
     const int c[3] = {i, j, k};
-    const long long m = (long long)i * g.st0 + p;
+    const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
     const long long n = (long long)s * g.cells + m;
     const long long st[3] = {g.st0, (long long)g.N[2], 1};
     const bool up[3] = {e_updated(g, 0, c), e_updated(g, 1, c), e_updated(g, 2, c)};
     const bool pml = has_cpml && in_any_face(g, true, c);
-
-    // block-local source entries (sorted by block, then source order)
-    const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    int sb = 0, se = 0;
-    if (src.count > 0) {
-        sb = src.block_off[blin];
-        se = src.block_off[blin + 1];
-    }
-    const bool in_box = hist.active && i >= hist.box.x0 && i < hist.box.x0 + hist.box.nx && j >= hist.box.y0 &&
-                        j < hist.box.y0 + hist.box.ny && k >= hist.box.z0 && k < hist.box.z0 + hist.box.nz;
+    const bool in_box = in_hist_box(hist, i, j, k);
     if (!(up[0] || up[1] || up[2] || pml || se > sb || in_box)) return;
 
     float e[3] = {E.c[0][n], E.c[1][n], E.c[2][n]};
@@ -184,22 +268,7 @@ __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__
     const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
     const float cbv = cb[m];
     if (!pml && plain_cell(g, Radius<ORDER>::value, c)) {
-        // every component updated, no clamped difference, no CPML term
-        {
-            const float dHz = dbwd_full<ORDER>(Hf.c[2], n, st[1]);
-            const float dHy = dbwd_full<ORDER>(Hf.c[1], n, 1);
-            e[0] = u0 * e[0] + uy * dHz - uz * dHy;
-        }
-        {
-            const float dHx = dbwd_full<ORDER>(Hf.c[0], n, 1);
-            const float dHz = dbwd_full<ORDER>(Hf.c[2], n, st[0]);
-            e[1] = u0 * e[1] + uz * dHx - u1 * dHz;
-        }
-        {
-            const float dHy = dbwd_full<ORDER>(Hf.c[1], n, st[0]);
-            const float dHx = dbwd_full<ORDER>(Hf.c[0], n, st[1]);
-            e[2] = u0 * e[2] + u1 * dHy - uy * dHx;
-        }
+        fwd_e_plain<ORDER>(g, Hf, n, u0, u1, uy, uz, e);
     } else {
         // each staggered difference once, shared by the curl update and the CPML terms
         bool need[6] = {up[0], up[0], up[1], up[1], up[2], up[2]};
@@ -249,34 +318,50 @@ __global__ __launch_bounds__(256) void k_fwd_e(Grid g, const float* __restrict__
             }
         }
     }
-    // Hertzian dipole sources of this block, applied in source order.
-    for (int q = sb; q < se; ++q) {
-        const int2 ent = src.entries[q];
-        if ((long long)ent.x == m) {
-            const float scale = src.wave[(long long)ent.y * src.nt + src.it] * src.dl / src.vol;
-            e[src.comp] -= cbv * scale;
+    fwd_e_finish(src, hist, sb, se, in_box, i, j, k, s, m, n, e, eold, up, u0, cbv, pml || se > sb, E);
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256, ORDER == 8 ? 5 : 8) void k_fwd_e(Grid g, const float* __restrict__ ca,
+                                               const float* __restrict__ ccurl,
+                                               const float* __restrict__ cb, CF3 Hf, F3 E, Cpml cp,
+                                               int has_cpml, SourceBlocks src, HistWrite hist, CellMap cm)
+{
+    const int s = blockIdx.y;
+    // block-local source entries (sorted by block, then source order)
+    const int blin = blockIdx.x + gridDim.x * blockIdx.y;
+    int i, j, k;
+    int idx;
+    const int part = sweep_part(cm, blockIdx.x, idx);
+    if (part < 0) return;
+    if (part == 1) {
+        // interior box: every cell plain and every component updated
+        if (!box_cell(cm, idx, threadIdx.x, i, j, k)) return;
+        const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
+        const long long n = (long long)s * g.cells + m;
+        int sb = 0, se = 0;
+        if (src.count > 0) {
+            sb = src.block_off[blin];
+            se = src.block_off[blin + 1];
         }
+        const bool up[3] = {true, true, true};
+        float e[3] = {E.c[0][n], E.c[1][n], E.c[2][n]};
+        const float eold[3] = {e[0], e[1], e[2]};
+        const float u0 = ca[m], u1 = ccurl[m];
+        const float uy = axis_coeff(g, u1, 1), uz = axis_coeff(g, u1, 2);
+        const float cbv = cb[m];
+        fwd_e_plain<ORDER>(g, Hf, n, u0, u1, uy, uz, e);
+        fwd_e_finish(src, hist, sb, se, in_hist_box(hist, i, j, k), i, j, k, s, m, n, e, eold, up, u0, cbv,
+                     se > sb, E);
+        return;
     }
-    // history: E^n (pre-update) and the effective right-hand side R^n
-    if (in_box) {
-        const long long h = ((long long)(i - hist.box.x0) * hist.box.ny + (j - hist.box.y0)) * hist.box.nz +
-                            (k - hist.box.z0);
-        const long long base = (long long)s * hist.box.cells + h;
-        const int comps = hist.mode == 3 ? 3 : 1;
-        for (int ci = 0; ci < comps; ++ci) {
-            const int comp = hist.mode == 3 ? ci : 2;
-            if (hist.save_e) store_hist(hist.e_base, ci * hist.e_cstride + base, eold[comp], hist.e_kind);
-            if (hist.save_r) {
-                const float a = up[comp] ? u0 : 1.0f;
-                const float rhs = cbv != 0.0f ? (e[comp] - a * eold[comp]) / cbv : 0.0f;
-                store_hist(hist.r_base, ci * hist.r_cstride + base, rhs, hist.r_kind);
-            }
-        }
+    if (!shell_cell(cm, idx, threadIdx.x, i, j, k)) return;
+    int sb = 0, se = 0;
+    if (src.count > 0) {
+        sb = src.block_off[blin];
+        se = src.block_off[blin + 1];
     }
-    const bool touched = pml || se > sb;
-    if (up[0] || touched) E.c[0][n] = e[0];
-    if (up[1] || touched) E.c[1][n] = e[1];
-    if (up[2] || touched) E.c[2][n] = e[2];
+    fwd_e_cell<ORDER>(g, ca, ccurl, cb, Hf, E, cp, has_cpml, src, hist, sb, se, i, j, k, s);
 }
 
 // Receiver gather: rx[s, it, r] = E_comp at the receiver (deepgpr_cpu.c:600-623)
@@ -514,6 +599,27 @@ __device__ __forceinline__ int chan_of(int u, int a) { return 2 * u + (a > u ? a
 // Channel weights of cell (c, shot s) from its lambda values lt3 (curl
 // coefficient cu; upd is the CPML source coefficient array): one store per
 // channel, and the cell's lambda psi advanced.
+// Channel weights of a cell with every component updated and no CPML term:
+// the curl terms alone.
+template <bool ELECTRIC>
+__device__ __forceinline__ void adj_weights_plain(const Grid& g, float cu, const float (&lt3)[3],
+                                                  float* __restrict__ W, long long n)
+{
+    const long long cstride = (long long)g.nshot * g.cells;
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a == u) continue;
+            const int t = 3 - u - a;
+            const float kk = axis_coeff(g, cu, a);
+            const int t1 = t == 2 ? 0 : t + 1;
+            const bool plus = (a == t1) == ELECTRIC;
+            W[chan_of(u, a) * cstride + n] = 0.0f + (plus ? kk : -kk) * lt3[t];
+        }
+    }
+}
+
 template <bool ELECTRIC>
 __device__ __forceinline__ void adj_weights_cell(const Grid& g, float cu, const float* __restrict__ upd,
                                                  const float (&lt3)[3], const Cpml& cp, float* __restrict__ W,
@@ -521,19 +627,7 @@ __device__ __forceinline__ void adj_weights_cell(const Grid& g, float cu, const 
 {
     const long long cstride = (long long)g.nshot * g.cells;
     if (!cpml && plain_cell(g, 1, c)) {
-        // every component updated and no CPML term: the curl terms alone
-#pragma unroll
-        for (int u = 0; u < 3; ++u) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (a == u) continue;
-                const int t = 3 - u - a;
-                const float kk = axis_coeff(g, cu, a);
-                const int t1 = t == 2 ? 0 : t + 1;
-                const bool plus = (a == t1) == ELECTRIC;
-                W[chan_of(u, a) * cstride + n] = 0.0f + (plus ? kk : -kk) * lt3[t];
-            }
-        }
+        adj_weights_plain<ELECTRIC>(g, cu, lt3, W, n);
         return;
     }
 #pragma unroll
@@ -581,108 +675,135 @@ __device__ __forceinline__ void adj_weights_cell(const Grid& g, float cu, const 
 }
 
 template <bool ELECTRIC>
-__global__ __launch_bounds__(256) void k_adj_weights(Grid g, const float* __restrict__ curl,
+__global__ __launch_bounds__(256, 8) void k_adj_weights(Grid g, const float* __restrict__ curl,
                                                      const float* __restrict__ upd, CF3 lam, Cpml cp,
-                                                     float* __restrict__ W, int has_cpml, ZRuns zr)
+                                                     float* __restrict__ W, int has_cpml, CellMap cm)
 {
-    int j, k;
-    if (!run_cell(g, zr, j, k)) return;
-    const int p = j * g.N[2] + k;
-    const int i = blockIdx.y, s = blockIdx.z;
+    const int s = blockIdx.y;
+    int i, j, k;
+    int idx;
+    const int part = sweep_part(cm, blockIdx.x, idx);
+    if (part < 0) return;
+    if (part == 1) {
+        if (!box_cell(cm, idx, threadIdx.x, i, j, k)) return;
+        const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
+        const long long n = (long long)s * g.cells + m;
+        const float lt3[3] = {lam.c[0][n], lam.c[1][n], lam.c[2][n]};
+        adj_weights_plain<ELECTRIC>(g, curl[m], lt3, W, n);
+        return;
+    }
+    if (!shell_cell(cm, idx, threadIdx.x, i, j, k)) return;
     const int c[3] = {i, j, k};
-    const long long m = (long long)i * g.st0 + p;
+    const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
     const long long n = (long long)s * g.cells + m;
     const bool cpml = has_cpml != 0 && in_any_face(g, ELECTRIC, c);
     const float lt3[3] = {lam.c[0][n], lam.c[1][n], lam.c[2][n]};
     adj_weights_cell<ELECTRIC>(g, curl[m], upd, lt3, cp, W, cpml, c, m, n, s);
 }
 
-// lambda_H += D-^T W_E (the E-side weights); lambda_E is scaled later.
-template <int ORDER>
-__global__ __launch_bounds__(256) void k_adj_gather_e(Grid g, const float* __restrict__ W, F3 lamH, ZRuns zr)
+// lambda_H[u] += D-^T W_E over the two axes a != u. MODE -2: a cell of the
+// interior box (every gather full-width); -1: any cell.
+template <int ORDER, int MODE>
+__device__ __forceinline__ float gather_e_comp(const Grid& g, const float* __restrict__ W, long long n,
+                                               const int (&c)[3], int u, float acc)
 {
-    int j, k;
-    if (!run_cell(g, zr, j, k)) return;
-    const int p = j * g.N[2] + k;
-    const int i = blockIdx.y, s = blockIdx.z;
-    const int c[3] = {i, j, k};
-    const long long n = (long long)s * g.cells + (long long)i * g.st0 + p;
     const long long st[3] = {g.st0, (long long)g.N[2], 1};
     const long long cstride = (long long)g.nshot * g.cells;
 #pragma unroll
-    for (int u = 2; u >= 0; --u) {
-        float acc = lamH.c[u][n];
+    for (int a = 0; a < 3; ++a) {
+        if (a == u) continue;
+        const float* Wc = W + chan_of(u, a) * cstride + n;
+        auto wf = [&](int off) { return Wc[off * st[a]]; };
+        if (MODE == -2) acc += gather_bwd_full<ORDER>(wf);
+        else acc += gather_bwd_T<ORDER>(wf, c[a], g.N[a]);
+    }
+    return acc;
+}
+
+// lambda_E[t] (already scaled by ca where updated) += D+^T W_H over a != t
+// (MODE as for gather_e_comp).
+template <int ORDER, int MODE>
+__device__ __forceinline__ float gather_h_comp(const Grid& g, const float* __restrict__ W, long long n,
+                                               const int (&c)[3], int t, float acc)
+{
+    const long long st[3] = {g.st0, (long long)g.N[2], 1};
+    const long long cstride = (long long)g.nshot * g.cells;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (a == u) continue;
-            const float* Wc = W + chan_of(u, a) * cstride + n;
-            auto wf = [&](int off) { return Wc[off * st[a]]; };
-            acc += gather_bwd_T<ORDER>(wf, c[a], g.N[a]);
-        }
+    for (int a = 0; a < 3; ++a) {
+        if (a == t) continue;
+        const float* Wc = W + chan_of(t, a) * cstride + n;
+        auto wf = [&](int off) { return Wc[off * st[a]]; };
+        if (MODE == -2) acc += gather_fwd_full<ORDER>(wf);
+        else acc += gather_fwd_T<ORDER>(wf, c[a], g.N[a]);
+    }
+    return acc;
+}
+
+// lambda_H += D-^T W_E (the E-side weights); lambda_E is scaled later.
+template <int ORDER>
+__global__ __launch_bounds__(256, ORDER == 8 ? 5 : 8) void k_adj_gather_e(Grid g, const float* __restrict__ W, F3 lamH, CellMap cm)
+{
+    const int s = blockIdx.y;
+    int i, j, k;
+    int idx;
+    const int part = sweep_part(cm, blockIdx.x, idx);
+    if (part < 0) return;
+    const bool boxed = part == 1;
+    if (boxed ? !box_cell(cm, idx, threadIdx.x, i, j, k) : !shell_cell(cm, idx, threadIdx.x, i, j, k)) return;
+    const int c[3] = {i, j, k};
+    const long long n = (long long)s * g.cells + (long long)i * g.st0 + (long long)j * g.N[2] + k;
+#pragma unroll
+    for (int u = 2; u >= 0; --u) {
+        const float acc = boxed ? gather_e_comp<ORDER, -2>(g, W, n, c, u, lamH.c[u][n])
+                                : gather_e_comp<ORDER, -1>(g, W, n, c, u, lamH.c[u][n]);
         lamH.c[u][n] = acc;
     }
 }
 
 // lambda_E = ca lambda_E (updated components) + D+^T W_H.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_adj_gather_h(Grid g, const float* __restrict__ ca,
-                                                      const float* __restrict__ W, F3 lamE, ZRuns zr)
+__global__ __launch_bounds__(256, ORDER == 8 ? 5 : 8) void k_adj_gather_h(Grid g, const float* __restrict__ ca,
+                                                      const float* __restrict__ W, F3 lamE, CellMap cm)
 {
-    int j, k;
-    if (!run_cell(g, zr, j, k)) return;
-    const int p = j * g.N[2] + k;
-    const int i = blockIdx.y, s = blockIdx.z;
+    const int s = blockIdx.y;
+    int i, j, k;
+    int idx;
+    const int part = sweep_part(cm, blockIdx.x, idx);
+    if (part < 0) return;
+    if (part == 1) {
+        if (!box_cell(cm, idx, threadIdx.x, i, j, k)) return;
+        const int c[3] = {i, j, k};
+        const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
+        const long long n = (long long)s * g.cells + m;
+        const float cav = ca[m];
+#pragma unroll
+        for (int t = 2; t >= 0; --t) lamE.c[t][n] = gather_h_comp<ORDER, -2>(g, W, n, c, t, cav * lamE.c[t][n]);
+        return;
+    }
+    if (!shell_cell(cm, idx, threadIdx.x, i, j, k)) return;
     const int c[3] = {i, j, k};
-    const long long m = (long long)i * g.st0 + p;
+    const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
     const long long n = (long long)s * g.cells + m;
-    const long long st[3] = {g.st0, (long long)g.N[2], 1};
-    const long long cstride = (long long)g.nshot * g.cells;
     const float cav = ca[m];
 #pragma unroll
     for (int t = 2; t >= 0; --t) {
         float acc = lamE.c[t][n];
         if (e_updated(g, t, c)) acc = cav * acc;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (a == t) continue;
-            const float* Wc = W + chan_of(t, a) * cstride + n;
-            auto wf = [&](int off) { return Wc[off * st[a]]; };
-            acc += gather_fwd_T<ORDER>(wf, c[a], g.N[a]);
-        }
-        lamE.c[t][n] = acc;
+        lamE.c[t][n] = gather_h_comp<ORDER, -1>(g, W, n, c, t, acc);
     }
 }
 
-// The E-side gather followed by the H-side weights of the same step, so that
-// the new lambda_H is used while it is in registers: lambda_H += D-^T W_E,
-// then W_H of the new lambda_H. W_E and W_H are separate arrays (every cell
-// reads W_E at its neighbours and writes only its own W_H channels).
+// A shell cell of k_adj_gather_e_w.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_adj_gather_e_w(Grid g, const float* __restrict__ We, F3 lamH, ZRuns zr,
-                                                        const float* __restrict__ hcurl,
-                                                        const float* __restrict__ hrhs, Cpml cph,
-                                                        float* __restrict__ Wh, int has_cpml)
+__device__ __forceinline__ void gather_e_w_cell(const Grid& g, const float* __restrict__ We, const F3& lamH,
+                                                const float* __restrict__ hcurl, const float* __restrict__ hrhs,
+                                                const Cpml& cph, float* __restrict__ Wh, int has_cpml,
+                                                const int (&c)[3], long long m, long long n, int s)
 {
-    int j, k;
-    if (!run_cell(g, zr, j, k)) return;
-    const int p = j * g.N[2] + k;
-    const int i = blockIdx.y, s = blockIdx.z;
-    const int c[3] = {i, j, k};
-    const long long m = (long long)i * g.st0 + p;
-    const long long n = (long long)s * g.cells + m;
-    const long long st[3] = {g.st0, (long long)g.N[2], 1};
-    const long long cstride = (long long)g.nshot * g.cells;
     float lt3[3];
 #pragma unroll
     for (int u = 2; u >= 0; --u) {
-        float acc = lamH.c[u][n];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (a == u) continue;
-            const float* Wc = We + chan_of(u, a) * cstride + n;
-            auto wf = [&](int off) { return Wc[off * st[a]]; };
-            acc += gather_bwd_T<ORDER>(wf, c[a], g.N[a]);
-        }
+        const float acc = gather_e_comp<ORDER, -1>(g, We, n, c, u, lamH.c[u][n]);
         lamH.c[u][n] = acc;
         lt3[u] = acc;
     }
@@ -690,56 +811,80 @@ __global__ __launch_bounds__(256) void k_adj_gather_e_w(Grid g, const float* __r
     adj_weights_cell<false>(g, hcurl[m], hrhs, lt3, cph, Wh, cpml, c, m, n, s);
 }
 
+// The E-side gather followed by the H-side weights of the same step, so that
+// the new lambda_H is used while it is in registers: lambda_H += D-^T W_E,
+// then W_H of the new lambda_H. W_E and W_H are separate arrays (every cell
+// reads W_E at its neighbours and writes only its own W_H channels).
+template <int ORDER>
+__global__ __launch_bounds__(256, ORDER == 8 ? 5 : 8) void k_adj_gather_e_w(Grid g, const float* __restrict__ We, F3 lamH, CellMap cm,
+                                                        const float* __restrict__ hcurl,
+                                                        const float* __restrict__ hrhs, Cpml cph,
+                                                        float* __restrict__ Wh, int has_cpml)
+{
+    const int s = blockIdx.y;
+    int i, j, k;
+    float lt3[3];
+    int idx;
+    const int part = sweep_part(cm, blockIdx.x, idx);
+    if (part < 0) return;
+    if (part == 1) {
+        if (!box_cell(cm, idx, threadIdx.x, i, j, k)) return;
+        const int c[3] = {i, j, k};
+        const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
+        const long long n = (long long)s * g.cells + m;
+#pragma unroll
+        for (int u = 2; u >= 0; --u) {
+            const float acc = gather_e_comp<ORDER, -2>(g, We, n, c, u, lamH.c[u][n]);
+            lamH.c[u][n] = acc;
+            lt3[u] = acc;
+        }
+        adj_weights_plain<false>(g, hcurl[m], lt3, Wh, n);
+        return;
+    }
+    if (!shell_cell(cm, idx, threadIdx.x, i, j, k)) return;
+    const int c[3] = {i, j, k};
+    const long long m = (long long)i * g.st0 + (long long)j * g.N[2] + k;
+    const long long n = (long long)s * g.cells + m;
+    gather_e_w_cell<ORDER>(g, We, lamH, hcurl, hrhs, cph, Wh, has_cpml, c, m, n, s);
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
 
-CellSweep cell_sweep(const Grid& g, int order, bool split)
+// The interior box runs its own plain-cell path (same bits as the general
+// path); DEEPGPR_AMD_3D_BOX=off sends every cell through the general path.
+// Read at every launch, so one process can run both.
+bool box_sweeps()
 {
-    // The z-interior of every row (no z-face CPML term, no clamped z
-    // difference) and the rest are the two parts of the launch, so that a
-    // wavefront of consecutive cells rarely mixes the interior and the CPML
-    // branches, while the CPML workgroups fill the CUs beside the interior
-    // ones instead of running as a second, latency-bound launch.
-    const int R = order / 2, nz = g.N[2];
-    int lo = std::max(R, g.pml[4] > 0 ? g.pml[4] + 1 : 0);
-    int hi = nz - 1 - R;
-    if (g.pml[5] > 0) hi = std::min(hi, nz - 2 - g.pml[5]);
+    const char* v = std::getenv("DEEPGPR_AMD_3D_BOX");
+    return !(v && std::strcmp(v, "off") == 0);
+}
+
+// The shell and box blocks are interleaved in the launch order, so the
+// latency-bound CPML blocks share the CUs with the bandwidth-bound box
+// blocks; DEEPGPR_AMD_3D_INTERLEAVE=off launches every shell block first
+// (same bits). Read at every launch.
+bool interleave_sweeps()
+{
+    const char* v = std::getenv("DEEPGPR_AMD_3D_INTERLEAVE");
+    return !(v && std::strcmp(v, "off") == 0);
+}
+
+CellSweep cell_sweep(const Grid& g, int order, bool use_box)
+{
     CellSweep cs{};
-    if (!split || lo > hi || hi - lo + 1 < 64) {
-        cs.zr.run[0] = ZRun{0, nz, 0, 0};
-        cs.zr.run[1] = ZRun{0, 0, 0, 0};
-    } else {
-        cs.zr.run[0] = ZRun{lo, hi - lo + 1, 0, 0};
-        cs.zr.run[1] = ZRun{0, lo, hi + 1, nz - 1 - hi};
-    }
-    const auto blocks_of = [&](const ZRun& z) {
-        return (long long)((long long)(z.na + z.nb) * g.N[1] + 255) / 256;
-    };
-    const long long b0 = blocks_of(cs.zr.run[0]);
-    unsigned bx = (unsigned)(b0 + blocks_of(cs.zr.run[1]));
-    // keep the x-planes of one column of blocks on the same XCD (8 XCDs,
-    // round-robin block placement) so x-neighbours are served by its L2; the
-    // padding workgroups fall in part 1 and exit
-    if (bx >= 8) bx = (bx + 7) / 8 * 8;
-    cs.zr.split = (int)b0;
-    cs.grid = dim3(bx, (unsigned)g.N[0], (unsigned)g.nshot);
-    cs.blocks = (long long)bx * g.N[0] * g.nshot;
+    cs.cm = make_cell_map(g.N, g.pml, order / 2, use_box && box_sweeps());
+    if (interleave_sweeps()) interleave(cs.cm);
+    cs.grid = dim3((unsigned)cs.cm.per_shot, (unsigned)g.nshot, 1u);
+    cs.blocks = (long long)cs.cm.per_shot * g.nshot;
     return cs;
 }
 
 long long cell_block(const CellSweep& cs, const Grid& g, int i, int j, int k, int s)
 {
-    for (int q = 0; q < 2; ++q) {
-        const ZRun& z = cs.zr.run[q];
-        int r = -1;
-        if (k >= z.za && k < z.za + z.na) r = k - z.za;
-        else if (k >= z.zb && k < z.zb + z.nb) r = z.na + (k - z.zb);
-        if (r < 0) continue;
-        const long long cell = (long long)j * (z.na + z.nb) + r;
-        return (q ? cs.zr.split : 0) + cell / 256 + (long long)cs.grid.x * (i + (long long)g.N[0] * s);
-    }
-    return -1;
+    const long long b = cell_map_block(cs.cm, i, j, k);
+    return b < 0 ? -1 : b + (long long)cs.cm.per_shot * s;
 }
 
 void launch_coefficients(const Grid& g, const float* eps, const float* sig, const float* mu, float* const co[6],
@@ -755,16 +900,16 @@ void launch_fwd_h(int order, const Grid& g, const StepArgs& a, hipStream_t st)
     const CellSweep cs = cell_sweep(g, order);
     {
         const dim3 grid = cs.grid;
-        const ZRuns zr = cs.zr;
+        const CellMap cm = cs.cm;
         if (order == 8)
             hipLaunchKernelGGL(k_fwd_h<8>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml,
-                               zr);
+                               cm);
         else if (order == 4)
             hipLaunchKernelGGL(k_fwd_h<4>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml,
-                               zr);
+                               cm);
         else
             hipLaunchKernelGGL(k_fwd_h<2>, grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.Ec, a.H, a.cph, a.has_cpml,
-                               zr);
+                               cm);
     }
 }
 
@@ -773,16 +918,16 @@ void launch_fwd_e(int order, const Grid& g, const StepArgs& a, hipStream_t st)
     const CellSweep cs = cell_sweep(g, order);
     {
         const dim3 grid = cs.grid;
-        const ZRuns zr = cs.zr;
+        const CellMap cm = cs.cm;
         if (order == 8)
             hipLaunchKernelGGL(k_fwd_e<8>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
-                               a.has_cpml, a.src, a.hist, zr);
+                               a.has_cpml, a.src, a.hist, cm);
         else if (order == 4)
             hipLaunchKernelGGL(k_fwd_e<4>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
-                               a.has_cpml, a.src, a.hist, zr);
+                               a.has_cpml, a.src, a.hist, cm);
         else
             hipLaunchKernelGGL(k_fwd_e<2>, grid, dim3(256), 0, st, g, a.ca, a.ccurl, a.cb, a.Hc, a.E, a.cpe,
-                               a.has_cpml, a.src, a.hist, zr);
+                               a.has_cpml, a.src, a.hist, cm);
     }
 }
 
@@ -855,14 +1000,14 @@ void launch_add_f64(const double* a, float* out, long long count, hipStream_t st
 
 void launch_adj_weights(int order, bool electric, const Grid& g, const AdjArgs& a, float* W, hipStream_t st)
 {
-    const CellSweep cs = cell_sweep(g, order, false);  // measured: the split does not pay here
+    const CellSweep cs = cell_sweep(g, order);
     {
         if (electric)
             hipLaunchKernelGGL(k_adj_weights<true>, cs.grid, dim3(256), 0, st, g, a.ccurl, a.cb, a.lEc, a.cpe, W,
-                               a.has_cpml, cs.zr);
+                               a.has_cpml, cs.cm);
         else
             hipLaunchKernelGGL(k_adj_weights<false>, cs.grid, dim3(256), 0, st, g, a.hcurl, a.hrhs, a.lHc, a.cph,
-                               W, a.has_cpml, cs.zr);
+                               W, a.has_cpml, cs.cm);
     }
 }
 
@@ -871,9 +1016,9 @@ void launch_adj_gather_e(int order, const Grid& g, const AdjArgs& a, const float
     const CellSweep cs = cell_sweep(g, order);
     {
         const dim3 grid = cs.grid;
-        if (order == 8) hipLaunchKernelGGL(k_adj_gather_e<8>, grid, dim3(256), 0, st, g, W, a.lH, cs.zr);
-        else if (order == 4) hipLaunchKernelGGL(k_adj_gather_e<4>, grid, dim3(256), 0, st, g, W, a.lH, cs.zr);
-        else hipLaunchKernelGGL(k_adj_gather_e<2>, grid, dim3(256), 0, st, g, W, a.lH, cs.zr);
+        if (order == 8) hipLaunchKernelGGL(k_adj_gather_e<8>, grid, dim3(256), 0, st, g, W, a.lH, cs.cm);
+        else if (order == 4) hipLaunchKernelGGL(k_adj_gather_e<4>, grid, dim3(256), 0, st, g, W, a.lH, cs.cm);
+        else hipLaunchKernelGGL(k_adj_gather_e<2>, grid, dim3(256), 0, st, g, W, a.lH, cs.cm);
     }
 }
 
@@ -882,10 +1027,10 @@ void launch_adj_gather_h(int order, const Grid& g, const AdjArgs& a, const float
     const CellSweep cs = cell_sweep(g, order);
     {
         const dim3 grid = cs.grid;
-        if (order == 8) hipLaunchKernelGGL(k_adj_gather_h<8>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.zr);
+        if (order == 8) hipLaunchKernelGGL(k_adj_gather_h<8>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.cm);
         else if (order == 4)
-            hipLaunchKernelGGL(k_adj_gather_h<4>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.zr);
-        else hipLaunchKernelGGL(k_adj_gather_h<2>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.zr);
+            hipLaunchKernelGGL(k_adj_gather_h<4>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.cm);
+        else hipLaunchKernelGGL(k_adj_gather_h<2>, grid, dim3(256), 0, st, g, a.ca, W, a.lE, cs.cm);
     }
 }
 
@@ -894,15 +1039,15 @@ void launch_adj_gather_e_w(int order, const Grid& g, const AdjArgs& a, const flo
     const CellSweep cs = cell_sweep(g, order);
     {
         const dim3 grid = cs.grid;
-        const ZRuns zr = cs.zr;
+        const CellMap cm = cs.cm;
         if (order == 8)
-            hipLaunchKernelGGL(k_adj_gather_e_w<8>, grid, dim3(256), 0, st, g, We, a.lH, zr, a.hcurl, a.hrhs, a.cph,
+            hipLaunchKernelGGL(k_adj_gather_e_w<8>, grid, dim3(256), 0, st, g, We, a.lH, cm, a.hcurl, a.hrhs, a.cph,
                                Wh, a.has_cpml);
         else if (order == 4)
-            hipLaunchKernelGGL(k_adj_gather_e_w<4>, grid, dim3(256), 0, st, g, We, a.lH, zr, a.hcurl, a.hrhs, a.cph,
+            hipLaunchKernelGGL(k_adj_gather_e_w<4>, grid, dim3(256), 0, st, g, We, a.lH, cm, a.hcurl, a.hrhs, a.cph,
                                Wh, a.has_cpml);
         else
-            hipLaunchKernelGGL(k_adj_gather_e_w<2>, grid, dim3(256), 0, st, g, We, a.lH, zr, a.hcurl, a.hrhs, a.cph,
+            hipLaunchKernelGGL(k_adj_gather_e_w<2>, grid, dim3(256), 0, st, g, We, a.lH, cm, a.hcurl, a.hrhs, a.cph,
                                Wh, a.has_cpml);
     }
 }
