@@ -291,7 +291,8 @@ def _main() -> int:
     args = ap.parse_args()
     reqs = synthetic.batch_instances(args.instances, seed=args.seed, nt_scale=args.scale)
     dev = torch.device("cuda:0")
-    run_sequential(reqs[:1], dev)  # warm the library and the allocator
+    # warm the library, the allocator and the tile tuner (one probe per shape)
+    run_sequential(reqs, dev)
     seq = run_sequential(reqs, dev)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()

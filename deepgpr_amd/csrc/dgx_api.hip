@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <new>
 #include <mutex>
@@ -530,23 +531,159 @@ bool xmarch3d()
     return v && std::strcmp(v, "xmarch") == 0;
 }
 
-// Tile rows of a single fused TM solve. Small solves leave most of the GPU
-// idle and their time is the chain of one workgroup, which shorter tiles cut:
-// kTmBITiny when its kTmBITiny-row tiles over all shots fit in one generation
-// of resident workgroups (kTmTinyGroups), else kTmBISmall when the kTmBI-row
-// tiles number fewer than kTmSmallTiles, else kTmBI.
-// DEEPGPR_AMD_TM2D_TILE_ROWS=4, 8 or 16 forces one; it is read on every call,
-// so one process can run all three.
-int tm2d_tile_rows(const Plan& pl)
+// Static rule for the tile rows of a single fused TM solve. Small solves
+// leave most of the GPU idle and their time is the chain of one workgroup,
+// which shorter tiles cut: kTmBITiny when its kTmBITiny-row tiles over all
+// shots fit in one generation of resident workgroups (kTmTinyGroups), else
+// kTmBISmall when the kTmBI-row tiles number fewer than kTmSmallTiles, else
+// kTmBI.
+int tm2d_tile_rows_rule(const Plan& pl)
+{
+    const long long cols = (pl.g.N[1] + kTmBJ - 1) / kTmBJ;
+    const auto tiles = [&](int bi) { return (long long)((pl.g.N[0] + bi - 1) / bi) * cols * pl.g.nshot; };
+    if (tiles(kTmBITiny) <= kTmTinyGroups) return kTmBITiny;
+    return tiles(kTmBI) < kTmSmallTiles ? kTmBISmall : kTmBI;
+}
+
+Tm2dGrid tm_grid_of(const Plan& pl, int bi);
+
+// Run-time tuner of the tile rows: the first solve of each shape (grid,
+// shots, order, CPML) times the fused forward step kernel at 4, 8 and 16
+// rows on buffers of its own (zeroed fields and CPML auxiliaries, the solve's
+// coefficients, no sources, receivers or history: 2 warm-up and 6 timed
+// steps each), and the fastest is kept for every later solve of that shape
+// in the process, forward and adjoint. The tile rows change the work
+// decomposition only, never the results. Shapes whose probe buffers would
+// exceed kTuneMaxBytes, and every shape with DEEPGPR_AMD_TM2D_TUNE=0, use the
+// static rule.
+constexpr size_t kTuneMaxBytes = (size_t)512 << 20;
+struct TuneKey {
+    int n0, n1, nshot, order, cpml;
+    bool operator==(const TuneKey& o) const
+    {
+        return n0 == o.n0 && n1 == o.n1 && nshot == o.nshot && order == o.order && cpml == o.cpml;
+    }
+};
+std::mutex g_tune_mu;
+std::vector<std::pair<TuneKey, int>> g_tuned;
+
+int tm2d_probe(const Plan& pl, const dgx_solve* P, hipStream_t stream, double* us_out)
+{
+    const int cand[3] = {kTmBITiny, kTmBISmall, kTmBI};
+    Tm2dGrid g0 = tm_grid_of(pl, kTmBI);
+    const long long field = (long long)g0.nshot * g0.plane;
+    long long psi_len = 0;
+    long long len_e[4], len_h[4];
+    for (int f4 = 0; f4 < 4; ++f4) {
+        const long long across = (long long)g0.nshot * (f4 < 2 ? g0.NY : g0.NX);
+        len_e[f4] = g0.pml[f4] > 0 ? across * (g0.pml[f4] + 1) : 0;
+        len_h[f4] = g0.pml[f4] > 0 ? across * g0.pml[f4] : 0;
+        psi_len += 2 * (len_e[f4] + len_h[f4]);
+    }
+    const size_t bytes = ((size_t)6 * field + 5 * (size_t)g0.plane + (size_t)psi_len) * sizeof(float);
+    if (bytes > kTuneMaxBytes) return -1;
+    DevBuf buf;
+    if (!buf.alloc(bytes)) return -1;
+    if (hipMemsetAsync(buf.p, 0, bytes, stream) != hipSuccess) return -1;
+    float* b = (float*)buf.p;
+    Tm2dFields f{};
+    TmPingPong pp{};
+    for (int q = 0; q < 2; ++q) {
+        pp.Ez[q] = b + (3 * q) * field;
+        pp.Hx[q] = b + (3 * q + 1) * field;
+        pp.Hy[q] = b + (3 * q + 2) * field;
+    }
+    float* co = b + 6 * field;
+    const float* full[5] = {P->update_coeff[0], P->update_coeff[1], P->update_coeff[2], P->update_coeff[4],
+                            P->update_coeff[5]};
+    for (int c = 0; c < 5; ++c) launch_tm_pack(full[c], co + (long long)c * g0.plane, g0.plane, false, stream);
+    f.Ez = pp.Ez[0];
+    f.Hx = pp.Hx[0];
+    f.Hy = pp.Hy[0];
+    f.ca = co;
+    f.ccurl = co + g0.plane;
+    f.cb = co + 2 * (long long)g0.plane;
+    f.hcurl = co + 3 * (long long)g0.plane;
+    f.hrhs = co + 4 * (long long)g0.plane;
+    float* a = co + 5 * (long long)g0.plane;
+    for (int f4 = 0; f4 < 4; ++f4) {
+        f.coef_e[f4] = P->pml_coeff[f4];
+        f.coef_h[f4] = P->pml_coeff[6 + f4];
+        for (int q = 0; q < 2; ++q) {
+            pp.psi_e[q][f4] = len_e[f4] ? a : nullptr;
+            a += len_e[f4];
+            pp.psi_h[q][f4] = len_h[f4] ? a : nullptr;
+            a += len_h[f4];
+        }
+        f.psi_e[f4] = pp.psi_e[0][f4];
+        f.psi_h[f4] = pp.psi_h[0][f4];
+    }
+    SourceBlocks src{};
+    TmReceiversFwd rx{};
+    HistWrite none{};
+    none.mode = pl.comps == 3 ? 3 : 2;
+    none.box = pl.box;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int best = -1;
+    double best_us = 0.0;
+    bool ok = hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess;
+    for (int q = 0; q < 3 && ok; ++q) {
+        const Tm2dGrid g = tm_grid_of(pl, cand[q]);
+        TmPingPong cur = pp;
+        auto step = [&](int it) {
+            launch_tm_step_fwd(pl.order, g, f, cur, src, rx, none, it, stream);
+            std::swap(cur.Ez[0], cur.Ez[1]);
+            std::swap(cur.Hx[0], cur.Hx[1]);
+            std::swap(cur.Hy[0], cur.Hy[1]);
+            for (int f4 = 0; f4 < 4; ++f4) {
+                std::swap(cur.psi_e[0][f4], cur.psi_e[1][f4]);
+                std::swap(cur.psi_h[0][f4], cur.psi_h[1][f4]);
+            }
+        };
+        step(0);
+        step(1);
+        ok = hipEventRecord(ev[0], stream) == hipSuccess;
+        for (int it = 2; it < 8 && ok; ++it) step(it);
+        ok = ok && hipEventRecord(ev[1], stream) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess;
+        float ms = 0.0f;
+        ok = ok && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess;
+        if (ok && (best < 0 || ms * 1e3 / 6 < best_us)) {
+            best = cand[q];
+            best_us = ms * 1e3 / 6;
+        }
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (!ok || hipGetLastError() != hipSuccess) return -1;
+    if (us_out) *us_out = best_us;
+    return best;
+}
+
+// Tile rows of a single fused TM solve: DEEPGPR_AMD_TM2D_TILE_ROWS=4, 8 or 16
+// forces one (read on every call, so one process can run all three); else
+// the tuned choice for the solve's shape.
+int tm2d_tile_rows(const Plan& pl, const dgx_solve* P, hipStream_t stream)
 {
     const char* v = std::getenv("DEEPGPR_AMD_TM2D_TILE_ROWS");
     if (v && std::strcmp(v, "4") == 0) return kTmBITiny;
     if (v && std::strcmp(v, "8") == 0) return kTmBISmall;
     if (v && std::strcmp(v, "16") == 0) return kTmBI;
-    const long long cols = (pl.g.N[1] + kTmBJ - 1) / kTmBJ;
-    const auto tiles = [&](int bi) { return (long long)((pl.g.N[0] + bi - 1) / bi) * cols * pl.g.nshot; };
-    if (tiles(kTmBITiny) <= kTmTinyGroups) return kTmBITiny;
-    return tiles(kTmBI) < kTmSmallTiles ? kTmBISmall : kTmBI;
+    const char* t = std::getenv("DEEPGPR_AMD_TM2D_TUNE");
+    if ((t && std::strcmp(t, "0") == 0) || !P) return tm2d_tile_rows_rule(pl);
+    const TuneKey key{pl.g.N[0], pl.g.N[1], pl.g.nshot, pl.order, pl.has_cpml ? 1 : 0};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        for (const auto& kv : g_tuned)
+            if (kv.first == key) return kv.second;
+    }
+    int bi = tm2d_probe(pl, P, stream, nullptr);
+    if (bi < 0) {
+        (void)hipGetLastError();
+        bi = tm2d_tile_rows_rule(pl);
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned.push_back({key, bi});
+    return bi;
 }
 
 Tm2dGrid tm_grid_of(const Plan& pl, int bi)
@@ -863,7 +1000,7 @@ int run_forward(const dgx_solve* P, hipStream_t stream, void* fixed = nullptr, s
         set_error("dgx_forward_noalloc: INT8 histories need a scratch frame; use dgx_forward");
         return -1;
     }
-    const Tm2dGrid tg = tm_grid_of(pl, tm2d_tile_rows(pl));
+    const Tm2dGrid tg = tm_grid_of(pl, fused ? tm2d_tile_rows(pl, P, stream) : kTmBI);
     const int tm_tiles = (g.N[0] * g.N[1] + 255) / 256;
     long long nblocks = tm ? (long long)tm_tiles * g.nshot
                            : (xm ? (long long)xt.nty * xt.ntz * g.nshot : sweep.blocks);
@@ -1248,7 +1385,7 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
 
     const bool tm = tm2d_applies(P, pl, hsloc, hrloc);
     const bool fused = tm && tm2d_fused();
-    const Tm2dGrid tg = tm_grid_of(pl, tm2d_tile_rows(pl));
+    const Tm2dGrid tg = tm_grid_of(pl, fused ? tm2d_tile_rows(pl, P, stream) : kTmBI);
     Grid gc = g;  // compact 2D TM addressing for the receiver/source kernels
     if (tm) {
         gc.st0 = g.N[1];
@@ -1625,14 +1762,86 @@ struct Batch {
     const TmBatchInst* desc[6] = {};
     const int* prefix[6] = {};
     int ninst[6] = {}, total[6] = {}, order_of[6] = {2, 4, 8, 2, 4, 8};
+    // kBatchGraphSteps steps of the current table as one graph (kernel nodes
+    // only); node q launches step *gbase + q
+    DevBuf gbase;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_failed = false;
     ~Batch()
     {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (Staging& r : ring) {
             if (r.done) (void)hipEventDestroy(r.done);
             if (r.host) (void)hipHostFree(r.host);
         }
     }
 };
+
+constexpr int kBatchGraphSteps = 32;
+
+// Groups of kBatchGraphSteps batch steps between phase ends replay one graph
+// per descriptor table; DEEPGPR_AMD_BATCH_GRAPHS=0 launches every step one by
+// one (same bits). Read at every call.
+bool batch_graphs()
+{
+    const char* v = std::getenv("DEEPGPR_AMD_BATCH_GRAPHS");
+    return !(v && std::strcmp(v, "0") == 0);
+}
+
+// The graph reads the descriptor table it was captured with: it is retired
+// (destroyed once the batch stream has passed its last launch) whenever the
+// table is rebuilt.
+void batch_drop_graph(Batch* b)
+{
+    if (!b->gexec && !b->graph) return;
+    hipEvent_t done = nullptr;
+    if (hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess &&
+        hipEventRecord(done, b->stream) == hipSuccess) {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        g_graphs.push_back({b->gexec, b->graph, done});
+    } else {
+        if (done) (void)hipEventDestroy(done);
+        (void)hipStreamSynchronize(b->stream);
+        if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+    }
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// Captures kBatchGraphSteps steps of the current table (not executed while
+// captured); false leaves the caller to launch the steps one by one.
+bool batch_graph_ready(Batch* b)
+{
+    if (b->gexec) return true;
+    if (b->graph_failed) return false;
+    hipStream_t cs = capture_stream();
+    if (!cs || (!b->gbase.p && !b->gbase.alloc(sizeof(int)))) return false;
+    const int* base = (const int*)b->gbase.p;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        for (int q = 0; q < kBatchGraphSteps; ++q)
+            for (int slot = 0; slot < 6; ++slot)
+                launch_tm_batch(b->order_of[slot], slot >= 3, b->desc[slot], b->prefix[slot], b->ninst[slot],
+                                b->total[slot], q, cs, base);
+        ok = hipStreamEndCapture(cs, &graph) == hipSuccess && graph;
+    }
+    ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (!ok) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();  // a failed capture falls back to launches one by one
+        b->graph_failed = true;
+        return false;
+    }
+    b->graph = graph;
+    b->gexec = exec;
+    return true;
+}
 
 int order_slot(int order) { return order == 8 ? 2 : (order == 4 ? 1 : 0); }
 
@@ -1907,15 +2116,31 @@ int batch_phase_end(Batch* b, BatchInstance& I, bool& finished)
 int batch_step(Batch* b, int steps)
 {
     CallStream call(b->stream);
-    for (int k = 0; k < steps && !b->active.empty(); ++k) {
-        if (b->dirty && batch_rebuild(b)) {
-            set_error("batch: out of device memory for descriptors");
-            return -1;
+    sweep_graphs();
+    for (int k = 0; k < steps && !b->active.empty();) {
+        if (b->dirty) {
+            batch_drop_graph(b);
+            if (batch_rebuild(b)) {
+                set_error("batch: out of device memory for descriptors");
+                return -1;
+            }
         }
-        for (int slot = 0; slot < 6; ++slot)
-            launch_tm_batch(b->order_of[slot], slot >= 3, b->desc[slot], b->prefix[slot], b->ninst[slot],
-                            b->total[slot], b->gstep, b->stream);
-        b->gstep += 1;
+        // steps before the first phase end: a whole graph of them replays
+        // kBatchGraphSteps steps at once, the phase ends are handled after it
+        long long room = std::numeric_limits<long long>::max();
+        for (const auto& ip : b->active) room = std::min<long long>(room, (long long)ip->d.start + ip->d.nt - b->gstep);
+        int adv = 1;
+        if (batch_graphs() && room >= kBatchGraphSteps && steps - k >= kBatchGraphSteps && batch_graph_ready(b)) {
+            launch_step_counter((int*)b->gbase.p, b->gstep, false, b->stream);
+            DGX_TRY(hipGraphLaunch(b->gexec, b->stream), "batch: graph launch");
+            adv = kBatchGraphSteps;
+        } else {
+            for (int slot = 0; slot < 6; ++slot)
+                launch_tm_batch(b->order_of[slot], slot >= 3, b->desc[slot], b->prefix[slot], b->ninst[slot],
+                                b->total[slot], b->gstep, b->stream);
+        }
+        b->gstep += adv;
+        k += adv;
         for (size_t q = 0; q < b->active.size();) {
             BatchInstance& I = *b->active[q];
             if (b->gstep - I.d.start < I.d.nt) {
@@ -1958,6 +2183,15 @@ DGX_API int dgx_forward_noalloc(const dgx_solve* solve, dgx_stream_t stream)
 }
 
 DGX_API long long dgx_forward_noalloc_scratch_bytes(void) { return (long long)kNoAllocScratchBytes; }
+
+DGX_API int dgx_tm2d_tile_rows_tuned(int nx_fields, int ny_fields, int nshot, int order, int cpml)
+{
+    const TuneKey key{nx_fields, ny_fields, nshot, order, cpml ? 1 : 0};
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const auto& kv : g_tuned)
+        if (kv.first == key) return kv.second;
+    return 0;
+}
 
 DGX_API int dgx_backward(const dgx_solve* solve, dgx_stream_t stream)
 {

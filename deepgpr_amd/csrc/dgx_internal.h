@@ -212,9 +212,10 @@ struct TmBatchInst {
 };
 
 // Batched step of every listed instance of one order and phase:
-// prefix[k] .. prefix[k+1] are the workgroups of instance k.
+// prefix[k] .. prefix[k+1] are the workgroups of instance k. The global step
+// is gstep, plus *gbase when gbase is set (the launches of a replayed graph).
 void launch_tm_batch(int order, bool adjoint, const TmBatchInst* insts, const int* prefix, int ninst,
-                     int total_groups, int gstep, hipStream_t st);
+                     int total_groups, int gstep, hipStream_t st, const int* gbase = nullptr);
 void launch_tm_final_frame(const Tm2dGrid& g, const float* ez, const HistWrite& hw, hipStream_t st);
 void launch_scale(const float* in, float* out, float factor, long long count, hipStream_t st);
 // Float arrays to zero in one launch (a fill kernel, not hipMemsetAsync).

@@ -255,8 +255,9 @@ __device__ __forceinline__ size_t hist_elem_bytes(int kind) { return kind == kF3
 template <int ORDER>
 __global__ __launch_bounds__(NTF) void k_tm_batch_fwd(const TmBatchInst* __restrict__ insts,
                                                       const int* __restrict__ prefix, int ninst, int total,
-                                                      int gstep)
+                                                      int gstep, const int* __restrict__ gbase)
 {
+    if (gbase) gstep += *gbase;  // a replayed graph: the step is the base plus the node's offset
     const int b = batch_group(total);
     if (b < 0) return;
     const int k = batch_instance(prefix, ninst, b);
@@ -288,8 +289,9 @@ __global__ __launch_bounds__(NTF) void k_tm_batch_fwd(const TmBatchInst* __restr
 template <int ORDER>
 __global__ __launch_bounds__(NTA) void k_tm_batch_adj(const TmBatchInst* __restrict__ insts,
                                                       const int* __restrict__ prefix, int ninst, int total,
-                                                      int gstep)
+                                                      int gstep, const int* __restrict__ gbase)
 {
+    if (gbase) gstep += *gbase;  // a replayed graph: the step is the base plus the node's offset
     const int b = batch_group(total);
     if (b < 0) return;
     const int k = batch_instance(prefix, ninst, b);
@@ -383,24 +385,24 @@ void launch_clear_spans(const ClearSpans& sp, hipStream_t st)
 }
 
 void launch_tm_batch(int order, bool adjoint, const TmBatchInst* insts, const int* prefix, int ninst,
-                     int total_groups, int gstep, hipStream_t st)
+                     int total_groups, int gstep, hipStream_t st, const int* gbase)
 {
     if (ninst <= 0 || total_groups <= 0) return;
     const dim3 grid((unsigned)((total_groups + 7) / 8 * 8));
     if (adjoint) {
         if (order == 8)
-            hipLaunchKernelGGL(k_tm_batch_adj<8>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep);
+            hipLaunchKernelGGL(k_tm_batch_adj<8>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep, gbase);
         else if (order == 4)
-            hipLaunchKernelGGL(k_tm_batch_adj<4>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep);
+            hipLaunchKernelGGL(k_tm_batch_adj<4>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep, gbase);
         else
-            hipLaunchKernelGGL(k_tm_batch_adj<2>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep);
+            hipLaunchKernelGGL(k_tm_batch_adj<2>, grid, dim3(NTA), 0, st, insts, prefix, ninst, total_groups, gstep, gbase);
     } else {
         if (order == 8)
-            hipLaunchKernelGGL(k_tm_batch_fwd<8>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep);
+            hipLaunchKernelGGL(k_tm_batch_fwd<8>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep, gbase);
         else if (order == 4)
-            hipLaunchKernelGGL(k_tm_batch_fwd<4>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep);
+            hipLaunchKernelGGL(k_tm_batch_fwd<4>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep, gbase);
         else
-            hipLaunchKernelGGL(k_tm_batch_fwd<2>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep);
+            hipLaunchKernelGGL(k_tm_batch_fwd<2>, grid, dim3(NTF), 0, st, insts, prefix, ninst, total_groups, gstep, gbase);
     }
 }
 

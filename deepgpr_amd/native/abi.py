@@ -124,6 +124,8 @@ NATIVE_FUNCTIONS: Dict[str, Tuple[tuple, object]] = {
     "dgx_backward": ((("solve", VOID_PTR), ("stream", VOID_PTR)), INT),
     "dgx_forward_noalloc": ((("solve", VOID_PTR), ("stream", VOID_PTR)), INT),
     "dgx_forward_noalloc_scratch_bytes": ((), LONGLONG),
+    "dgx_tm2d_tile_rows_tuned": ((("nx_fields", INT), ("ny_fields", INT), ("nshot", INT), ("order", INT),
+                                  ("cpml", INT)), INT),
     "dgx_codec_roundtrip": ((("input", FLOAT_PTR), ("encoded", VOID_PTR), ("decoded", FLOAT_PTR),
                              ("count", LONGLONG), ("storage_kind", INT),
                              ("conversion_backend", INT), ("stream", VOID_PTR)), INT),

@@ -123,6 +123,13 @@ DGX_API int dgx_backward(const dgx_solve* solve, dgx_stream_t stream);
  * share the array and race. */
 DGX_API int dgx_forward_noalloc(const dgx_solve* solve, dgx_stream_t stream);
 DGX_API long long dgx_forward_noalloc_scratch_bytes(void);
+/* No counterpart in the reference. Tile rows (4, 8 or 16) that the run-time
+ * tuner chose for the fused 2D Ez-TM step kernels of one shape (field grid
+ * nx_fields x ny_fields, shots, FDTD order, CPML present), or 0 when no solve
+ * of that shape has run yet in this process. The first solve of a shape times
+ * the forward step kernel at each height on buffers of its own and keeps the
+ * fastest; every height gives the same results. */
+DGX_API int dgx_tm2d_tile_rows_tuned(int nx_fields, int ny_fields, int nshot, int order, int cpml);
 /* FP16/BF16 history codec round trip on device arrays; replaces
  * deepgpr_test_wavefield_conversion (deepgpr.h:84-86). */
 DGX_API int dgx_codec_roundtrip(const float* input, void* encoded, float* decoded, long long count,
