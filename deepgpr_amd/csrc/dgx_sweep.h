@@ -132,8 +132,10 @@ DGX_HD int seg_of(const CellMap& cm, int b)
 // Shell block b (0 <= b < shell_end), thread t: its cell, or false.
 DGX_HD bool shell_cell(const CellMap& cm, int b, int t, int& i, int& j, int& k)
 {
-    // the block's segment, read in place (a uniform index into the argument block)
-    const SweepSeg& sg = cm.seg[seg_of(cm, b)];
+    // the block's segment by selects (an indexed read of the argument block
+    // can make the compiler copy the table to scratch memory)
+    const int si = seg_of(cm, b);
+    const SweepSeg sg = si == 2 ? cm.seg[2] : (si == 1 ? cm.seg[1] : cm.seg[0]);
     const unsigned L = (unsigned)(b - sg.first);
     const int pl = (int)div_q(L, sg.bpp_div);
     if (pl >= sg.np[0] + sg.np[1]) return false;
