@@ -23,6 +23,7 @@ from .solver.pml import (  # noqa: F401
     CFS, CFSParameter, build_pml_coeffs, build_pml_phi, buildpmlcoeffs, calculate_pml_update_coeffs, e0, m0,
 )
 from .solver.sampling import recommended_sampling_interval, source_max_frequency  # noqa: F401
+from .solver.segments import plan_segments, segmented_compute  # noqa: F401
 from .solver.storage import decompress_wavefield_history  # noqa: F401
 from .utils.exceptions import (  # noqa: F401
     CFLConditionError, DeepGPRError, NativeLibraryError, NativeLibraryNotFoundError, NonFiniteTensorError,

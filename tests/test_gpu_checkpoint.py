@@ -83,3 +83,41 @@ def test_checkpoint_segments_match_the_full_solve(cuda_device, shape, pml, reent
             model.clamp_(min=1.0)
             sigma.clamp_(min=0.0)
     assert not torch.equal(model, base)
+
+
+@pytest.mark.parametrize("shape,pml,segment_steps", [
+    ((8, 9), (2, 3, 1, 2), 12), ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12), ((30, 26, 34), (4, 5, 3, 4, 5, 3), None),
+])
+def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segment_steps):
+    """The library's segment driver: traces and final states bit for bit, and
+    gradients within float32 summation round-off, against one compute();
+    segment_steps None plans the segments from a memory budget a third of the
+    whole solve's need."""
+    import deepgpr_amd as dg
+    print(f"[gpu checkpoint] segmented_compute {shape} segment_steps={segment_steps}", flush=True)
+    base, sigma_base, args = _problem(shape, pml, cuda_device)
+    ref_model = base.clone().requires_grad_()
+    ref_sigma = sigma_base.clone().requires_grad_()
+    full = dg.compute(eps_r=ref_model, sigma=ref_sigma, **args)
+    full[-1].square().sum().backward()
+
+    model = base.clone().requires_grad_()
+    sigma = sigma_base.clone().requires_grad_()
+    kw = {k: v for k, v in args.items() if k != "device"}
+    budget = None
+    if segment_steps is None:
+        # the driver's own estimate (E-only FP32 history at S 1, FP32 sums) of a third of the steps
+        n = [s + p0 + p1 for s, p0, p1 in zip(shape, pml[0::2], pml[1::2])]
+        nt = args["source_amplitudes"].shape[1]
+        opts = dict(mode=args["mode"], reconstruct_rhs=True, fp64_accumulators=False, nrx=2)
+        budget = dg.estimate_compute_memory(*n, nt // 3, 2, list(pml), **opts)["total"]
+        planned = dg.plan_segments(nt, budget, n, 2, list(pml), **opts)
+        assert planned < nt, planned  # several segments
+    out = dg.segmented_compute(cuda_device, eps_r=model, sigma=sigma, segment_steps=segment_steps,
+                               memory_budget=budget, **kw)
+    assert torch.equal(out[-1], full[-1])
+    for a, b in zip((*out[1], *out[2], *out[3]), (*full[1], *full[2], *full[3])):
+        assert torch.equal(a, b)
+    out[-1].square().sum().backward()
+    for a, b in ((model.grad, ref_model.grad), (sigma.grad, ref_sigma.grad)):
+        torch.testing.assert_close(a, b, rtol=2.0e-5, atol=max(1.0e-5, 1.0e-7 * float(b.abs().max())))
