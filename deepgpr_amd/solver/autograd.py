@@ -9,6 +9,8 @@ never modified.
 """
 from __future__ import annotations
 
+import contextlib
+import contextvars
 from dataclasses import dataclass
 from typing import Any, Dict, List, Optional, Sequence, Tuple
 
@@ -117,12 +119,81 @@ def _check_capabilities(lib: Any, config: SolverConfig) -> None:
             raise NativeLibraryError(f"The loaded HIP library lacks {cap}; rebuild it.")
 
 
-def _final_frame(config: SolverConfig) -> torch.Tensor:
-    shape = saved_history_shape(config.mode, 1, config.nstep, *config.history_cells)
-    if config.storage.compression == "int8":
-        n = int8_history_layout(shape, config.storage.block_size)["packed_bytes"]  # type: ignore[arg-type]
-        return torch.empty(n, device=config.device, dtype=torch.int8)
-    return torch.empty(shape, device=config.device, dtype=config.storage.dtype)
+class HistoryPool:
+    """One device buffer that solves run one after another take their
+    histories from, instead of allocating them each time.
+
+    Successive histories of the same size then reuse the same bytes: no
+    release and re-allocation of blocks of up to hundreds of GB between the
+    solves (which costs seconds each on this device) and no caching-allocator
+    blocks split by smaller allocations in between. Each solve's histories
+    are views of the same bytes, so a solve may take them only once the
+    previous solve's backward has been queued on the same stream; the
+    segment driver, which re-runs one segment at a time inside its
+    checkpointed backward, is the user it is made for. The buffer grows to
+    the largest request and the pool holds it while `keep` is true; with
+    `keep` false the next take() hands out its views and lets go of it.
+
+    On a GPU the buffer lives in a private pool of torch's caching allocator
+    (torch.cuda.MemPool), so no other allocation is ever carved from it:
+    once the pool has let go of the buffer and the views are gone, its
+    memory is released whole at the allocator's next empty_cache() or
+    out-of-memory retry instead of staying pinned by small tensors placed
+    in the freed block. A new buffer that does not fit the device's free
+    memory empties the allocator's cache first."""
+
+    ALIGN = 256
+
+    def __init__(self, device):
+        self.device = torch.device(device)
+        self.buffer: Optional[torch.Tensor] = None
+        self.keep = True
+        self._mem = None  # the private allocator pool of the current buffer
+
+    def _allocate(self, nbytes: int) -> torch.Tensor:
+        if self.device.type != "cuda":
+            return torch.empty(nbytes, device=self.device, dtype=torch.uint8)
+        if torch.cuda.mem_get_info(self.device)[0] < nbytes:
+            # an allocation routed to a private pool does not make the
+            # allocator release its cache when it fails, so release it first
+            torch.cuda.empty_cache()
+        self._mem = torch.cuda.MemPool()  # a new pool per buffer: the old one is released with its buffer
+        with torch.cuda.use_mem_pool(self._mem, self.device):
+            return torch.empty(nbytes, device=self.device, dtype=torch.uint8)
+
+    def take(self, specs: Sequence[Tuple[Tuple[int, ...], torch.dtype]]) -> List[torch.Tensor]:
+        sizes = []
+        for shape, dtype in specs:
+            n = 1
+            for d in shape:
+                n *= int(d)
+            sizes.append(n * torch.empty(0, dtype=dtype).element_size())
+        offsets, total = [], 0
+        for n in sizes:
+            offsets.append(total)
+            total += -(-n // self.ALIGN) * self.ALIGN
+        if self.buffer is None or self.buffer.numel() < total:
+            self.buffer = self._mem = None  # release the smaller buffer before the larger one is allocated
+            self.buffer = self._allocate(max(total, 1))
+        views = [self.buffer[o:o + n].view(dtype).view(shape)
+                 for o, n, (shape, dtype) in zip(offsets, sizes, specs)]
+        if not self.keep:
+            self.buffer = self._mem = None
+        return views
+
+
+_HISTORY_POOL: contextvars.ContextVar[Optional[HistoryPool]] = contextvars.ContextVar("deepgpr_amd_history_pool",
+                                                                                     default=None)
+
+
+@contextlib.contextmanager
+def history_pool(pool: Optional[HistoryPool]):
+    """Solves started inside the block take their histories from `pool`."""
+    token = _HISTORY_POOL.set(pool)
+    try:
+        yield pool
+    finally:
+        _HISTORY_POOL.reset(token)
 
 
 def _allocate_histories(config: SolverConfig, needs_model_gradient: bool, reconstruct_rhs: bool):
@@ -130,27 +201,36 @@ def _allocate_histories(config: SolverConfig, needs_model_gradient: bool, recons
 
     Histories always stay resident in device memory (use_async_offload is
     accepted for API compatibility; 288 GB of HBM holds what the reference
-    staged to pinned host memory)."""
+    staged to pinned host memory). Inside a history_pool block they are views
+    of the pool's buffer."""
     dev, st = config.device, config.storage
     nt_saved = saved_time_steps(config.nt, config.sampling_interval)
     shape = saved_history_shape(config.mode, nt_saved, config.nstep, *config.history_cells)
     if not config.save_wavefield_history:
         return torch.empty(0, device=dev, dtype=st.dtype), torch.empty(0, device=dev, dtype=st.dtype), shape
-    final = _final_frame(config) if (reconstruct_rhs and needs_model_gradient) else None
     if st.compression == "int8":
         n = int8_history_layout(shape, st.block_size)["packed_bytes"]  # type: ignore[arg-type]
-        e_saved = torch.empty(n, device=dev, dtype=torch.int8)
-        if final is not None:
-            return e_saved, final, shape
-        r_saved = (torch.empty(n, device=dev, dtype=torch.int8) if needs_model_gradient
-                   else torch.empty(0, device=dev, dtype=torch.int8))
-        return e_saved, r_saved, shape
-    e_saved = torch.empty(shape, device=dev, dtype=st.dtype)
-    if final is not None:
-        return e_saved, final, shape
-    r_saved = (torch.empty(shape, device=dev, dtype=st.dtype) if needs_model_gradient
-               else torch.empty(0, device=dev, dtype=st.dtype))
-    return e_saved, r_saved, shape
+        frame: Tuple[Tuple[int, ...], torch.dtype] = ((n,), torch.int8)
+    else:
+        frame = (tuple(shape), st.dtype)
+    final = reconstruct_rhs and needs_model_gradient
+    specs = [frame]
+    if final:
+        fshape = saved_history_shape(config.mode, 1, config.nstep, *config.history_cells)
+        if st.compression == "int8":
+            specs.append(((int8_history_layout(fshape, st.block_size)["packed_bytes"],), torch.int8))
+        else:
+            specs.append((tuple(fshape), st.dtype))
+    elif needs_model_gradient:
+        specs.append(frame)
+    pool = _HISTORY_POOL.get()
+    if pool is not None and pool.device == torch.device(dev):
+        got = pool.take(specs)
+    else:
+        got = [torch.empty(sh, device=dev, dtype=dt) for sh, dt in specs]
+    if len(got) == 1:
+        got.append(torch.empty(0, device=dev, dtype=frame[1]))
+    return got[0], got[1], shape
 
 
 def check_tensors_for_nan_inf(d: str = "", **tensors: torch.Tensor) -> None:

@@ -4,9 +4,11 @@ history is alive at a time (the pattern of the reference's
 tests/checkpoint_example.ipynb cell 5 and tests/test_external_pml.py:145-191,
 as a library function).
 
-The forward runs the segments in order, keeping only their boundary states
-(E, H and the 24 CPML auxiliaries). The backward of each segment re-runs its
-forward with the history, then its exact adjoint. Receivers and final states
+The forward runs the segments in order without autograd, keeping only their
+boundary states (E, H and the 24 CPML auxiliaries). The backward of each
+segment re-runs its forward with the history, then its exact adjoint, so at
+most one segment's history is in memory at a time; the re-runs take their
+histories from one HistoryPool buffer. Receivers and final states
 equal those of one unsegmented compute() bit for bit; the eps_r, sigma and
 source gradients equal its gradients up to float32 summation order (autograd
 adds per-segment sums where the unsegmented adjoint sums all steps in one
@@ -24,6 +26,7 @@ from torch.utils.checkpoint import checkpoint
 from ..config.defaults import (DEFAULT_FDTD_ORDER, DEFAULT_GRADIENT_ACCUMULATOR, DEFAULT_GRADIENT_MODE,
                                DEFAULT_GRADIENT_SAMPLING_INTERVAL, DEFAULT_PML_THICKNESS)
 from ..preprocessing.model_setup import initialization
+from .autograd import HistoryPool, history_pool
 from .fields import checkpoint_initial_field
 from .memory import estimate_compute_memory
 from .modeling import compute
@@ -68,10 +71,11 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
 
     segment_steps: steps per segment (rounded up to a multiple of the gradient
     sampling interval); by default the longest segment whose solve fits
-    `memory_budget` bytes (default: 85 % of the device's free memory), from
+    `memory_budget` bytes (default: 60 % of the device's free memory,
+    counting what torch's allocator holds unused as free), from
     estimate_compute_memory. Other keywords go to compute() unchanged (E, H
     and PML initial states are not accepted: the run starts from zero states,
-    as checkpoint_initial_field gives them).
+    as checkpoint_initial_field gives them; nor is save_forward_wavefield_path).
 
     Returns (None, E, H, PML, receivers) like compute(): there is no single
     history to return.
@@ -79,6 +83,9 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
     for key in ("E", "H", "PML"):
         if compute_kwargs.get(key) is not None:
             raise ValueError(f"segmented_compute starts from zero states; {key} is not accepted")
+    if compute_kwargs.get("save_forward_wavefield_path") is not None:
+        raise ValueError("segmented_compute keeps no single history to save; save_forward_wavefield_path "
+                         "is not accepted")
     if model_gradient_sampling_interval == "auto":
         raise ValueError("segmented_compute needs an explicit model_gradient_sampling_interval")
     device = torch.device(device)
@@ -88,7 +95,13 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
         prep = initialization(device, eps_r, sigma, mu_r, source_amplitudes, source_location, receiver_location, dx,
                               dt, pmlthick, fdtd_order)
         if memory_budget is None:
-            memory_budget = int(0.85 * torch.cuda.mem_get_info(device)[0])
+            # memory the caching allocator holds unused is released to an
+            # allocation that needs it, so it counts as free; 60 % of that
+            # leaves room for what the backward keeps beside one segment's
+            # solve: every segment's boundary states, the gradients
+            free = torch.cuda.mem_get_info(device)[0]
+            cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+            memory_budget = int(0.6 * (free + cached))
         dtype = compute_kwargs.get("wavefield_storage_dtype", torch.float32)
         compression = compute_kwargs.get("wavefield_compression", "none") or "none"
         rhs = compute_kwargs.get("wavefield_rhs_history", "auto")
@@ -108,16 +121,33 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
     kwargs = dict(compute_kwargs, pmlthick=pmlthick, fdtd_order=fdtd_order,
                   model_gradient_sampling_interval=model_gradient_sampling_interval)
 
-    def segment(er, se, wave, *boundary):
+    # every re-run takes its history from one buffer sized for the longest
+    # segment, so the backward allocates the history once; the backward
+    # re-runs the first segment last, and that re-run lets the pool go of
+    # the buffer, which is freed when that segment's backward is done
+    pool = HistoryPool(device)
+
+    # only the backward's re-run (grad enabled) needs the history; the
+    # forward pass stores none (histories stay in HBM, so offload is moot)
+    forward_pass = dict(kwargs, save_wavefield_history=False, use_async_offload=False)
+
+    def segment(first, er, se, wave, *boundary):
+        pool.keep = not first
         work = tuple(t.clone() for t in boundary)
-        r = compute(device, dx, dt, wave, source_location, receiver_location, eps_r=er, sigma=se, mu_r=mu_r,
-                    E=work[:3], H=work[3:6], PML=work[6:], **kwargs)
+        with history_pool(pool):
+            r = compute(device, dx, dt, wave, source_location, receiver_location, eps_r=er, sigma=se, mu_r=mu_r,
+                        E=work[:3], H=work[3:6], PML=work[6:],
+                        **(kwargs if torch.is_grad_enabled() else forward_pass))
         return (*r[1], *r[2], *r[3], r[-1])
 
+    # Re-entrant checkpoints run each segment's forward without autograd, so
+    # nothing keeps that segment's history (the autograd function holds it as
+    # context, which the non-re-entrant form's saved-tensor hooks do not
+    # release); the backward re-runs one segment at a time with its history.
     traces = []
     for start in range(0, nt, segment_steps):
-        out = checkpoint(segment, eps_r, sigma, source_amplitudes[:, start:start + segment_steps], *state,
-                         use_reentrant=False)
+        out = checkpoint(segment, start == 0, eps_r, sigma, source_amplitudes[:, start:start + segment_steps], *state,
+                         use_reentrant=True)
         state, data = out[:-1], out[-1]
         traces.append(data)
     return None, tuple(state[0:3]), tuple(state[3:6]), tuple(state[6:30]), torch.cat(traces, dim=1)

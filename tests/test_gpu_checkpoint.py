@@ -85,17 +85,26 @@ def test_checkpoint_segments_match_the_full_solve(cuda_device, shape, pml, reent
     assert not torch.equal(model, base)
 
 
-@pytest.mark.parametrize("shape,pml,segment_steps", [
-    ((8, 9), (2, 3, 1, 2), 12), ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12), ((30, 26, 34), (4, 5, 3, 4, 5, 3), None),
+@pytest.mark.parametrize("shape,pml,segment_steps,history", [
+    ((8, 9), (2, 3, 1, 2), 12, None), ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, None),
+    ((30, 26, 34), (4, 5, 3, 4, 5, 3), None, None),
+    ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, "bfloat16"), ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, "int8"),
+    ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, "sampled"),
 ])
-def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segment_steps):
+def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segment_steps, history, monkeypatch):
     """The library's segment driver: traces and final states bit for bit, and
     gradients within float32 summation round-off, against one compute();
     segment_steps None plans the segments from a memory budget a third of the
-    whole solve's need."""
+    whole solve's need. The lossy histories (BF16 and INT8 E+R, which the
+    re-runs take from the driver's history pool as two buffers) store the same
+    frames as the unsegmented solve, and so do histories sampled every second
+    step, so their gradients meet the same bound."""
     import deepgpr_amd as dg
-    print(f"[gpu checkpoint] segmented_compute {shape} segment_steps={segment_steps}", flush=True)
+    print(f"[gpu checkpoint] segmented_compute {shape} segment_steps={segment_steps} history={history}", flush=True)
     base, sigma_base, args = _problem(shape, pml, cuda_device)
+    args.update({None: {}, "bfloat16": {"wavefield_storage_dtype": torch.bfloat16},
+                 "int8": {"wavefield_compression": "int8"},
+                 "sampled": {"model_gradient_sampling_interval": 2}}[history])
     ref_model = base.clone().requires_grad_()
     ref_sigma = sigma_base.clone().requires_grad_()
     full = dg.compute(eps_r=ref_model, sigma=ref_sigma, **args)
@@ -113,6 +122,15 @@ def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segme
         budget = dg.estimate_compute_memory(*n, nt // 3, 2, list(pml), **opts)["total"]
         planned = dg.plan_segments(nt, budget, n, 2, list(pml), **opts)
         assert planned < nt, planned  # several segments
+    from deepgpr_amd.solver import segments
+    pools = []
+
+    class Recorded(segments.HistoryPool):
+        def __init__(self, device):
+            super().__init__(device)
+            pools.append(self)
+
+    monkeypatch.setattr(segments, "HistoryPool", Recorded)
     out = dg.segmented_compute(cuda_device, eps_r=model, sigma=sigma, segment_steps=segment_steps,
                                memory_budget=budget, **kw)
     assert torch.equal(out[-1], full[-1])
@@ -121,3 +139,5 @@ def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segme
     out[-1].square().sum().backward()
     for a, b in ((model.grad, ref_model.grad), (sigma.grad, ref_sigma.grad)):
         torch.testing.assert_close(a, b, rtol=2.0e-5, atol=max(1.0e-5, 1.0e-7 * float(b.abs().max())))
+    # the re-runs took their histories from the pool, which let go of them after the backward
+    assert len(pools) == 1 and pools[0].buffer is None and not pools[0].keep

@@ -4,6 +4,7 @@ import pytest
 import torch
 
 from deepgpr_amd import estimate_compute_memory, plan_segments, segmented_compute
+from deepgpr_amd.solver.autograd import HistoryPool, history_pool
 
 C3 = dict(grid=(288, 288, 224), nstep=4, pml=[16] * 6)
 
@@ -32,3 +33,63 @@ def test_segmented_compute_refuses_initial_states():
     z = torch.zeros(1)
     with pytest.raises(ValueError):
         segmented_compute("cpu", 0.02, 1e-11, z, z, z, eps_r=z, sigma=z, E=(z, z, z))
+    with pytest.raises(ValueError):
+        segmented_compute("cpu", 0.02, 1e-11, z, z, z, eps_r=z, sigma=z, save_forward_wavefield_path="/tmp/x")
+
+
+def test_history_pool_hands_out_aligned_views_of_one_buffer():
+    pool = HistoryPool("cpu")
+    a, b = pool.take([((3, 5, 7), torch.float32), ((33,), torch.int8)])
+    assert a.shape == (3, 5, 7) and a.dtype == torch.float32 and a.is_contiguous()
+    assert b.shape == (33,) and b.dtype == torch.int8
+    buf = pool.buffer
+    assert a.untyped_storage().data_ptr() == buf.untyped_storage().data_ptr() == b.untyped_storage().data_ptr()
+    assert (b.data_ptr() - buf.data_ptr()) % HistoryPool.ALIGN == 0 and b.data_ptr() - a.data_ptr() >= a.nbytes
+    a.fill_(1.5)
+    b.fill_(-3)  # the two views do not overlap
+    assert bool((a == 1.5).all()) and bool((b == -3).all())
+    # a request no larger reuses the buffer; a larger one replaces it
+    c, = pool.take([((4, 4), torch.bfloat16)])
+    assert pool.buffer is buf and c.data_ptr() == buf.data_ptr()
+    d, e = pool.take([((40, 40), torch.float32), ((40, 40), torch.float32)])
+    assert pool.buffer is not buf and pool.buffer.numel() >= d.nbytes + e.nbytes
+    assert e.data_ptr() - d.data_ptr() >= d.nbytes
+    # keep false: the views still share one buffer, the pool no longer holds it
+    pool.keep = False
+    f, g = pool.take([((8,), torch.float32), ((8,), torch.float32)])
+    assert pool.buffer is None and f.untyped_storage().data_ptr() == g.untyped_storage().data_ptr()
+
+
+def _config(storage_dtype=torch.float32, compression="none", block=None, reconstruct=False):
+    from deepgpr_amd.solver.autograd import SolverConfig
+    from deepgpr_amd.solver.storage import WavefieldStorageConfig
+    st = WavefieldStorageConfig(storage_dtype, compression, block, "auto", "auto")
+    return SolverConfig(device=torch.device("cpu"), dtype=torch.float32, grid_spacing=(0.02, 0.02, 0.02), dt=1e-11,
+                        nx=12, ny=10, nz=8, nt=9, nstep=2, nsr=1, nrx=1, pmlthick=torch.zeros(6, dtype=torch.int64),
+                        source_direction=2, receiver_component=2, sampling_interval=2, storage=st,
+                        save_wavefield_history=True, use_async_offload=False, fdtd_order=2, mode=3, debug=False,
+                        reconstruct_rhs=reconstruct)
+
+
+@pytest.mark.parametrize("fmt", ["float32 E+R", "float32 E-only", "bfloat16 E+R", "int8 E+R", "int8 E-only"])
+def test_histories_from_the_pool_match_allocated_ones(fmt):
+    """_allocate_histories inside a history_pool block returns tensors of the
+    shapes and dtypes it allocates outside one, as views of the pool's buffer."""
+    from deepgpr_amd.solver.autograd import _allocate_histories
+    dtype = torch.bfloat16 if fmt.startswith("bfloat16") else torch.float32
+    int8 = fmt.startswith("int8")
+    cfg = _config(dtype, "int8" if int8 else "none", (4, 4, 4) if int8 else None)
+    e_only = fmt.endswith("E-only")
+    plain = _allocate_histories(cfg, True, e_only)
+    pool = HistoryPool("cpu")
+    with history_pool(pool):
+        pooled = _allocate_histories(cfg, True, e_only)
+    assert plain[2] == pooled[2]
+    for a, b in zip(plain[:2], pooled[:2]):
+        assert a.shape == b.shape and a.dtype == b.dtype and a.numel() > 0
+        assert b.untyped_storage().data_ptr() == pool.buffer.untyped_storage().data_ptr()
+    if e_only:  # one-frame final-E buffer
+        assert pooled[1].numel() * 5 == pooled[0].numel() or int8
+    # without a model gradient: no R buffer, and outside the block no pool
+    e, r, _ = _allocate_histories(cfg, False, False)
+    assert r.numel() == 0 and e.untyped_storage().data_ptr() != pool.buffer.untyped_storage().data_ptr()
