@@ -90,6 +90,10 @@ def test_histories_from_the_pool_match_allocated_ones(fmt):
         assert b.untyped_storage().data_ptr() == pool.buffer.untyped_storage().data_ptr()
     if e_only:  # one-frame final-E buffer
         assert pooled[1].numel() * 5 == pooled[0].numel() or int8
+    # a pool made for another device is not used
+    with history_pool(HistoryPool("meta")):
+        other = _allocate_histories(cfg, True, e_only)
+    assert other[0].device.type == "cpu"
     # without a model gradient: no R buffer, and outside the block no pool
     e, r, _ = _allocate_histories(cfg, False, False)
     assert r.numel() == 0 and e.untyped_storage().data_ptr() != pool.buffer.untyped_storage().data_ptr()
