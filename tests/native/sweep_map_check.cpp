@@ -103,6 +103,8 @@ int main()
     const int none[6] = {0, 0, 0, 0, 0, 0};
     const int uneven[6] = {5, 4, 3, 6, 4, 5};
     const int thin[6] = {1, 2, 2, 1, 1, 2};
+    const int t1[6] = {5, 4, 3, 6, 4, 5};   // grids of the GPU sweep-map test
+    const int t2[6] = {4, 5, 3, 4, 6, 5};
     for (int ilv = 0; ilv < 2; ++ilv) {
         run_case(289, 289, 225, c3, 2, true, ilv);    // the 3D benchmark model, order 4
         run_case(129, 129, 129, c3, 2, true, ilv);
@@ -114,6 +116,9 @@ int main()
         run_case(30, 26, 34, uneven, 2, true, ilv);   // too small for a box
         run_case(7, 8, 6, thin, 1, true, ilv);
         run_case(40, 33, 2, none, 2, true, ilv);      // one z-plane of cells
+        for (int R = 1; R <= 4; R *= 2) run_case(60, 56, 70, t1, R, true, ilv);
+        run_case(56, 52, 70, none, 2, true, ilv);
+        run_case(60, 40, 270, t2, 2, true, ilv);
     }
     if (failures) {
         std::printf("%d checks failed\n", failures);
