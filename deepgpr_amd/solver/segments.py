@@ -5,10 +5,10 @@ tests/checkpoint_example.ipynb cell 5 and tests/test_external_pml.py:145-191,
 as a library function).
 
 The forward runs the segments in order without autograd, keeping only their
-boundary states (E, H and the 24 CPML auxiliaries). The backward of each
-segment re-runs its forward with the history, then its exact adjoint, so at
-most one segment's history is in memory at a time; the re-runs take their
-histories from one HistoryPool buffer. Receivers and final states
+boundary states (E, H and the 24 CPML auxiliaries), except the last, which
+it runs with its history. The backward of each segment re-runs its forward
+with the history, then its exact adjoint, so at most one segment's history
+is in memory at a time; the histories all live in one HistoryPool buffer. Receivers and final states
 equal those of one unsegmented compute() bit for bit; the eps_r, sigma and
 source gradients equal its gradients up to float32 summation order (autograd
 adds per-segment sums where the unsegmented adjoint sums all steps in one
@@ -60,6 +60,20 @@ def plan_segments(nt: int, budget_bytes: int, grid, nstep: int, pml, *, mode: in
         else:
             hi = mid - 1
     return min(nt, lo * S)
+
+
+def segment_bounds(nt: int, segment_steps: int, sampling_interval: int = 1):
+    """[start, end) of each segment of nt steps. Every segment starts on a
+    multiple of the sampling interval, so the saved steps are those of the
+    unsegmented solve. The remainder segment goes first when that keeps the
+    rule, so that the last segment, where the backward starts and which the
+    forward runs with its history instead of the backward re-running it, is
+    a full one."""
+    S = max(int(sampling_interval), 1)
+    rem = nt % segment_steps
+    if rem and rem % S == 0:
+        return [(0, rem)] + [(a, a + segment_steps) for a in range(rem, nt, segment_steps)]
+    return [(a, min(a + segment_steps, nt)) for a in range(0, nt, segment_steps)]
 
 
 def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_location=None,
@@ -121,14 +135,15 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
     kwargs = dict(compute_kwargs, pmlthick=pmlthick, fdtd_order=fdtd_order,
                   model_gradient_sampling_interval=model_gradient_sampling_interval)
 
-    # every re-run takes its history from one buffer sized for the longest
-    # segment, so the backward allocates the history once; the backward
-    # re-runs the first segment last, and that re-run lets the pool go of
-    # the buffer, which is freed when that segment's backward is done
+    # the last segment's run and every re-run take their history from one
+    # buffer, so the solve allocates a history once; the backward re-runs
+    # the first segment last, and that re-run lets the pool go of the
+    # buffer, which is freed when that segment's backward is done
     pool = HistoryPool(device)
 
-    # only the backward's re-run (grad enabled) needs the history; the
-    # forward pass stores none (histories stay in HBM, so offload is moot)
+    # only runs with grad enabled (the last segment's, the backward's
+    # re-runs) store a history; the checkpointed forward runs store none
+    # (histories stay in HBM, so offload is moot)
     forward_pass = dict(kwargs, save_wavefield_history=False, use_async_offload=False)
 
     def segment(first, er, se, wave, *boundary):
@@ -140,14 +155,20 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
                         **(kwargs if torch.is_grad_enabled() else forward_pass))
         return (*r[1], *r[2], *r[3], r[-1])
 
-    # Re-entrant checkpoints run each segment's forward without autograd, so
-    # nothing keeps that segment's history (the autograd function holds it as
-    # context, which the non-re-entrant form's saved-tensor hooks do not
+    bounds = segment_bounds(nt, segment_steps, S)
+    wants_grad = torch.is_grad_enabled() and any(
+        getattr(t, "requires_grad", False) for t in (eps_r, sigma, source_amplitudes))
+    # Re-entrant checkpoints run each other segment's forward without
+    # autograd, so nothing keeps its history (the autograd function holds it
+    # as context, which the non-re-entrant form's saved-tensor hooks do not
     # release); the backward re-runs one segment at a time with its history.
     traces = []
-    for start in range(0, nt, segment_steps):
-        out = checkpoint(segment, start == 0, eps_r, sigma, source_amplitudes[:, start:start + segment_steps], *state,
-                         use_reentrant=True)
+    for q, (a, b) in enumerate(bounds):
+        wave = source_amplitudes[:, a:b]
+        if q == len(bounds) - 1 and wants_grad:
+            out = segment(q == 0, eps_r, sigma, wave, *state)
+        else:
+            out = checkpoint(segment, q == 0, eps_r, sigma, wave, *state, use_reentrant=True)
         state, data = out[:-1], out[-1]
         traces.append(data)
     return None, tuple(state[0:3]), tuple(state[3:6]), tuple(state[6:30]), torch.cat(traces, dim=1)

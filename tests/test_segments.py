@@ -4,6 +4,7 @@ import pytest
 import torch
 
 from deepgpr_amd import estimate_compute_memory, plan_segments, segmented_compute
+from deepgpr_amd.solver.segments import segment_bounds
 from deepgpr_amd.solver.autograd import HistoryPool, history_pool
 
 C3 = dict(grid=(288, 288, 224), nstep=4, pml=[16] * 6)
@@ -97,3 +98,18 @@ def test_histories_from_the_pool_match_allocated_ones(fmt):
     # without a model gradient: no R buffer, and outside the block no pool
     e, r, _ = _allocate_histories(cfg, False, False)
     assert r.numel() == 0 and e.untyped_storage().data_ptr() != pool.buffer.untyped_storage().data_ptr()
+
+
+@pytest.mark.parametrize("nt,steps,S", [(2000, 388, 4), (35, 12, 1), (35, 12, 2), (36, 12, 3), (10, 12, 1),
+                                        (2001, 388, 4), (41, 8, 4)])
+def test_segment_bounds_cover_the_axis_on_sampling_starts(nt, steps, S):
+    b = segment_bounds(nt, steps, S)
+    assert b[0][0] == 0 and b[-1][1] == nt
+    assert all(x[1] == y[0] for x, y in zip(b, b[1:]))  # contiguous
+    assert all(a % S == 0 for a, _ in b)  # the unsegmented solve's saved steps
+    assert all(0 < e - a <= steps for a, e in b)
+    rem = nt % steps
+    # the last segment is a full one whenever the remainder can go first
+    if rem and rem % S == 0 and nt > steps:
+        assert b[0][1] - b[0][0] == rem and b[-1][1] - b[-1][0] == steps
+    assert segment_bounds(2000, 388, 4) == [(0, 60)] + [(60 + 388 * q, 448 + 388 * q) for q in range(5)]
