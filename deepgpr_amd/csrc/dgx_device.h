@@ -135,6 +135,93 @@ __device__ __forceinline__ float dfwd(const float* __restrict__ f, IDX id, typen
     }
 }
 
+// Branch-free forms of dbwd / dfwd for the general (CPML shell) path of the
+// 3D sweeps. With live, each tap the fitted stencil uses is loaded from the
+// address dbwd / dfwd load, and the other taps read the cell itself;
+// without live every tap reads the cell itself (the caller discards the
+// result). Every candidate radius's sum is formed in the reference's order
+// and the fitted one selected, so the same bits come out, while a wavefront
+// issues all its loads at once instead of one round trip per branch.
+// x, with its value hidden from the optimiser (an empty asm on a vector
+// register, no instruction): a load at base + opaque(c ? off : 0) stays one
+// unconditional load instead of becoming a load of base plus a masked load
+// of base + off, which would wait for the first before branching.
+__device__ __forceinline__ int opaque(int x)
+{
+    asm("" : "+v"(x));
+    return x;
+}
+
+template <int ORDER>
+__device__ __forceinline__ float dbwd_sel(const float* __restrict__ f, long long id, long long st, int c, int n,
+                                          bool live)
+{
+    if constexpr (ORDER <= 2) {
+        return f[id] - f[id - opaque(live ? 1 : 0) * st];
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        int rr = c < n - c ? c : n - c;  // fit_bwd: the largest r <= R with r <= c and r <= n - c, at least 1
+        rr = rr > R ? R : (rr < 1 ? 1 : rr);
+        float d[R];
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+            const bool use = live && r <= rr;
+            const int hi = opaque(use ? r - 1 : 0), lo = opaque(use ? r : 0);
+            d[r - 1] = f[id + hi * st] - f[id - lo * st];
+        }
+        float out = 0.0f;
+#pragma unroll
+        for (int q = 1; q <= R; ++q) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int r = 1; r <= q; ++r) acc += stencil_c(q, r) * d[r - 1];
+            out = rr == q ? acc : out;
+        }
+        return out;
+    }
+}
+
+template <int ORDER>
+__device__ __forceinline__ float dfwd_sel(const float* __restrict__ f, long long id, long long st, int c, int n,
+                                          bool live)
+{
+    if constexpr (ORDER <= 2) {
+        return f[id + opaque(live ? 1 : 0) * st] - f[id];
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        int rr = c + 1 < n - 1 - c ? c + 1 : n - 1 - c;  // fit_fwd: r <= c + 1 and r <= n - 1 - c
+        rr = rr > R ? R : (rr < 1 ? 1 : rr);
+        float d[R];
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+            const bool use = live && r <= rr;
+            const int hi = opaque(use ? r : 0), lo = opaque(use ? r - 1 : 0);
+            d[r - 1] = f[id + hi * st] - f[id - lo * st];
+        }
+        float out = 0.0f;
+#pragma unroll
+        for (int q = 1; q <= R; ++q) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int r = 1; r <= q; ++r) acc += stencil_c(q, r) * d[r - 1];
+            out = rr == q ? acc : out;
+        }
+        return out;
+    }
+}
+
+__device__ __forceinline__ int fit_bwd_sel(int c, int n, int R)
+{
+    int r = c < n - c ? c : n - c;
+    return r > R ? R : (r < 1 ? 1 : r);
+}
+
+__device__ __forceinline__ int fit_fwd_sel(int c, int n, int R)
+{
+    int r = c + 1 < n - 1 - c ? c + 1 : n - 1 - c;
+    return r > R ? R : (r < 1 ? 1 : r);
+}
+
 // The unclamped differences (the full-radius branch of dbwd / dfwd), for
 // cells at least R away from every face.
 template <int ORDER, typename IDX = long long>
@@ -259,6 +346,84 @@ __device__ __forceinline__ float gather_fwd_T(WF wf, int c, int n)
                 const int rs = fit_fwd(sm, n, R);
                 if (r <= rs) acc -= stencil_c(rs, r) * wf(r - 1);
             }
+        }
+        return acc;
+    }
+}
+
+// Branch-free forms of gather_bwd_T / gather_fwd_T for the general path of
+// the 3D sweeps: every tap is loaded (a tap outside the transposed stencil
+// reads offset 0), and each term is added or subtracted through a select in
+// the reference's order, which keeps the bits, the sign of a zero sum
+// included. Inside the full-width range every tap is used with the full
+// radius, the arithmetic of gather_*_full.
+template <int ORDER, typename WF>
+__device__ __forceinline__ float gather_bwd_sel(WF wf, int c, int n)
+{
+    if constexpr (ORDER <= 2) {
+        const bool ok = c + 1 < n;
+        const float w1 = wf(opaque(ok ? 1 : 0));
+        float acc = wf(0);
+        return ok ? acc - w1 : acc;
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float wp[R], wm[R];
+        bool okp[R], okm[R];
+        int rsp[R], rsm[R];
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+            const int sp = c - (r - 1), sm = c + r;
+            rsp[r - 1] = fit_bwd_sel(sp, n, R);
+            rsm[r - 1] = fit_bwd_sel(sm, n, R);
+            okp[r - 1] = sp >= 0 && r <= rsp[r - 1];
+            okm[r - 1] = sm < n && r <= rsm[r - 1];
+            wp[r - 1] = wf(opaque(okp[r - 1] ? -(r - 1) : 0));
+            wm[r - 1] = wf(opaque(okm[r - 1] ? r : 0));
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+            const float up = acc + stencil_c(rsp[r - 1], r) * wp[r - 1];
+            acc = okp[r - 1] ? up : acc;
+            const float dn = acc - stencil_c(rsm[r - 1], r) * wm[r - 1];
+            acc = okm[r - 1] ? dn : acc;
+        }
+        return acc;
+    }
+}
+
+template <int ORDER, typename WF>
+__device__ __forceinline__ float gather_fwd_sel(WF wf, int c, int n)
+{
+    if constexpr (ORDER <= 2) {
+        const bool ok = c >= 1;
+        const float w = wf(opaque(ok ? -1 : 0));
+        const float w0 = wf(0);
+        float acc = 0.0f;
+        acc = ok ? acc + w : acc;
+        return acc - w0;
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float wp[R], wm[R];
+        bool okp[R], okm[R];
+        int rsp[R], rsm[R];
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+            const int sp = c - r, sm = c + (r - 1);
+            rsp[r - 1] = fit_fwd_sel(sp, n, R);
+            rsm[r - 1] = fit_fwd_sel(sm, n, R);
+            okp[r - 1] = sp >= 0 && r <= rsp[r - 1];
+            okm[r - 1] = sm < n && r <= rsm[r - 1];
+            wp[r - 1] = wf(opaque(okp[r - 1] ? -r : 0));
+            wm[r - 1] = wf(opaque(okm[r - 1] ? r - 1 : 0));
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 1; r <= R; ++r) {
+            const float up = acc + stencil_c(rsp[r - 1], r) * wp[r - 1];
+            acc = okp[r - 1] ? up : acc;
+            const float dn = acc - stencil_c(rsm[r - 1], r) * wm[r - 1];
+            acc = okm[r - 1] ? dn : acc;
         }
         return acc;
     }

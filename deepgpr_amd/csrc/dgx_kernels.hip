@@ -117,12 +117,17 @@ __device__ __forceinline__ void fwd_h_cell(const Grid& g, const float* __restric
     }
     float dv[6];
 #pragma unroll
-    for (int d = 0; d < 6; ++d)
-        dv[d] = need[d] ? dfwd<ORDER>(E.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]], g.N[kDiffAxis[d]])
-                        : 0.0f;
-    if (up0) h[0] = h[0] - uy * dv[0] + uz * dv[1];  // dEz/dy, dEy/dz
-    if (up1) h[1] = h[1] - uz * dv[2] + u1 * dv[3];  // dEx/dz, dEz/dx
-    if (up2) h[2] = h[2] - u1 * dv[4] + uy * dv[5];  // dEy/dx, dEx/dy
+    for (int d = 0; d < 6; ++d) {
+        // a difference the cell does not need reads the cell itself and is not used
+        dv[d] = dfwd_sel<ORDER>(E.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]], g.N[kDiffAxis[d]], need[d]);
+    }
+    // selects rather than branches, so that every load above is issued at once
+    const float n0 = h[0] - uy * dv[0] + uz * dv[1];  // dEz/dy, dEy/dz
+    const float n1 = h[1] - uz * dv[2] + u1 * dv[3];  // dEx/dz, dEz/dx
+    const float n2 = h[2] - u1 * dv[4] + uy * dv[5];  // dEy/dx, dEx/dy
+    h[0] = up0 ? n0 : h[0];
+    h[1] = up1 ? n1 : h[1];
+    h[2] = up2 ? n2 : h[2];
     if (pml) {
         const float upd = hrhs[m];
 #pragma unroll
@@ -284,13 +289,18 @@ __device__ __forceinline__ void fwd_e_cell(const Grid& g, const float* __restric
         }
         float dv[6];
 #pragma unroll
-        for (int d = 0; d < 6; ++d)
-            dv[d] = need[d] ? dbwd<ORDER>(Hf.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]],
-                                          g.N[kDiffAxis[d]])
-                            : 0.0f;
-        if (up[0]) e[0] = u0 * e[0] + uy * dv[0] - uz * dv[1];  // dHz/dy, dHy/dz
-        if (up[1]) e[1] = u0 * e[1] + uz * dv[2] - u1 * dv[3];  // dHx/dz, dHz/dx
-        if (up[2]) e[2] = u0 * e[2] + u1 * dv[4] - uy * dv[5];  // dHy/dx, dHx/dy
+        for (int d = 0; d < 6; ++d) {
+            // a difference the cell does not need reads the cell itself and is not used
+            dv[d] = dbwd_sel<ORDER>(Hf.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]], g.N[kDiffAxis[d]],
+                                    need[d]);
+        }
+        // selects rather than branches, so that every load above is issued at once
+        const float n0 = u0 * e[0] + uy * dv[0] - uz * dv[1];  // dHz/dy, dHy/dz
+        const float n1 = u0 * e[1] + uz * dv[2] - u1 * dv[3];  // dHx/dz, dHz/dx
+        const float n2 = u0 * e[2] + u1 * dv[4] - uy * dv[5];  // dHy/dx, dHx/dy
+        e[0] = up[0] ? n0 : e[0];
+        e[1] = up[1] ? n1 : e[1];
+        e[2] = up[2] ? n2 : e[2];
         if (pml) {
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
@@ -715,7 +725,7 @@ __device__ __forceinline__ float gather_e_comp(const Grid& g, const float* __res
         const float* Wc = W + chan_of(u, a) * cstride + n;
         auto wf = [&](int off) { return Wc[off * st[a]]; };
         if (MODE == -2) acc += gather_bwd_full<ORDER>(wf);
-        else acc += gather_bwd_T<ORDER>(wf, c[a], g.N[a]);
+        else acc += gather_bwd_sel<ORDER>(wf, c[a], g.N[a]);
     }
     return acc;
 }
@@ -734,7 +744,7 @@ __device__ __forceinline__ float gather_h_comp(const Grid& g, const float* __res
         const float* Wc = W + chan_of(t, a) * cstride + n;
         auto wf = [&](int off) { return Wc[off * st[a]]; };
         if (MODE == -2) acc += gather_fwd_full<ORDER>(wf);
-        else acc += gather_fwd_T<ORDER>(wf, c[a], g.N[a]);
+        else acc += gather_fwd_sel<ORDER>(wf, c[a], g.N[a]);
     }
     return acc;
 }
