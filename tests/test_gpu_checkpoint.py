@@ -141,3 +141,40 @@ def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segme
         torch.testing.assert_close(a, b, rtol=2.0e-5, atol=max(1.0e-5, 1.0e-7 * float(b.abs().max())))
     # the re-runs took their histories from the pool, which let go of them after the backward
     assert len(pools) == 1 and pools[0].buffer is None and not pools[0].keep
+
+
+def test_segmented_compute_on_the_c3_grid(cuda_device):
+    """The segment driver at C3's real grid (256x256x192 + CPML 16, order 4,
+    all four shots in one call, sampling interval 4, FP32 E+R history, FP32
+    accumulators), shortened to nt 200 so that the unsegmented solve it is
+    checked against (about 89 GB of history) still fits: a 30 GB budget
+    plans several segments, whose re-runs take their histories from the
+    driver's pool. Traces and final states bit for bit, gradients within
+    float32 summation round-off."""
+    import deepgpr_amd as dg
+    from deepgpr_amd import synthetic
+    print("[gpu checkpoint] segmented_compute, C3 grid, 4 shots, nt 200", flush=True)
+    w = synthetic.c3_layered_3d(nshot=4, nt=200)
+    kw = dict(pmlthick=w.pml, fdtd_order=w.fdtd_order, mode=w.mode,
+              model_gradient_sampling_interval=w.sampling_interval, gradient_accumulator="float32", **w.kwargs)
+    src, rxl, amp = (w.source_location.to(cuda_device), w.receiver_location.to(cuda_device),
+                     w.source_amplitudes.to(cuda_device))
+    budget = 30 * 10**9
+    eps = w.eps_r.to(cuda_device).requires_grad_(True)
+    sig = w.sigma.to(cuda_device).requires_grad_(True)
+    out = dg.segmented_compute(cuda_device, w.dx, w.dt, amp, src, rxl, eps_r=eps, sigma=sig, memory_budget=budget,
+                               **kw)
+    out[-1].square().sum().backward()
+    seg = (out[-1].detach().clone(), [t.detach().clone() for t in (*out[1], *out[2])], eps.grad.clone(),
+           sig.grad.clone())
+    del out
+    torch.cuda.empty_cache()
+    eps2 = w.eps_r.to(cuda_device).requires_grad_(True)
+    sig2 = w.sigma.to(cuda_device).requires_grad_(True)
+    full = dg.compute(cuda_device, w.dx, w.dt, amp, src, rxl, eps_r=eps2, sigma=sig2, **kw)
+    full[-1].square().sum().backward()
+    assert torch.equal(seg[0], full[-1])
+    for a, b in zip(seg[1], (*full[1], *full[2])):
+        assert torch.equal(a, b)
+    for a, b in ((seg[2], eps2.grad), (seg[3], sig2.grad)):
+        torch.testing.assert_close(a, b, rtol=2.0e-5, atol=max(1.0e-5, 1.0e-7 * float(b.abs().max())))
