@@ -104,6 +104,8 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
         raise ValueError("segmented_compute needs an explicit model_gradient_sampling_interval")
     device = torch.device(device)
     nt = int(source_amplitudes.shape[1])
+    if nt < 1:
+        raise ValueError("segmented_compute needs at least one time step")
     S = int(model_gradient_sampling_interval)
     if segment_steps is None:
         prep = initialization(device, eps_r, sigma, mu_r, source_amplitudes, source_location, receiver_location, dx,

@@ -36,6 +36,8 @@ def test_segmented_compute_refuses_initial_states():
         segmented_compute("cpu", 0.02, 1e-11, z, z, z, eps_r=z, sigma=z, E=(z, z, z))
     with pytest.raises(ValueError):
         segmented_compute("cpu", 0.02, 1e-11, z, z, z, eps_r=z, sigma=z, save_forward_wavefield_path="/tmp/x")
+    with pytest.raises(ValueError, match="at least one time step"):
+        segmented_compute("cpu", 0.02, 1e-11, torch.zeros(1, 0, 1), z, z, eps_r=z, sigma=z)
 
 
 def test_history_pool_hands_out_aligned_views_of_one_buffer():
