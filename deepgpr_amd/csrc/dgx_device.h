@@ -135,13 +135,6 @@ __device__ __forceinline__ float dfwd(const float* __restrict__ f, IDX id, typen
     }
 }
 
-// Branch-free forms of dbwd / dfwd for the general (CPML shell) path of the
-// 3D sweeps. With live, each tap the fitted stencil uses is loaded from the
-// address dbwd / dfwd load, and the other taps read the cell itself;
-// without live every tap reads the cell itself (the caller discards the
-// result). Every candidate radius's sum is formed in the reference's order
-// and the fitted one selected, so the same bits come out, while a wavefront
-// issues all its loads at once instead of one round trip per branch.
 // x, with its value hidden from the optimiser (an empty asm on a vector
 // register, no instruction): a load at base + opaque(c ? off : 0) stays one
 // unconditional load instead of becoming a load of base plus a masked load
@@ -150,64 +143,6 @@ __device__ __forceinline__ int opaque(int x)
 {
     asm("" : "+v"(x));
     return x;
-}
-
-template <int ORDER>
-__device__ __forceinline__ float dbwd_sel(const float* __restrict__ f, long long id, long long st, int c, int n,
-                                          bool live)
-{
-    if constexpr (ORDER <= 2) {
-        return f[id] - f[id - opaque(live ? 1 : 0) * st];
-    } else {
-        constexpr int R = Radius<ORDER>::value;
-        int rr = c < n - c ? c : n - c;  // fit_bwd: the largest r <= R with r <= c and r <= n - c, at least 1
-        rr = rr > R ? R : (rr < 1 ? 1 : rr);
-        float d[R];
-#pragma unroll
-        for (int r = 1; r <= R; ++r) {
-            const bool use = live && r <= rr;
-            const int hi = opaque(use ? r - 1 : 0), lo = opaque(use ? r : 0);
-            d[r - 1] = f[id + hi * st] - f[id - lo * st];
-        }
-        float out = 0.0f;
-#pragma unroll
-        for (int q = 1; q <= R; ++q) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int r = 1; r <= q; ++r) acc += stencil_c(q, r) * d[r - 1];
-            out = rr == q ? acc : out;
-        }
-        return out;
-    }
-}
-
-template <int ORDER>
-__device__ __forceinline__ float dfwd_sel(const float* __restrict__ f, long long id, long long st, int c, int n,
-                                          bool live)
-{
-    if constexpr (ORDER <= 2) {
-        return f[id + opaque(live ? 1 : 0) * st] - f[id];
-    } else {
-        constexpr int R = Radius<ORDER>::value;
-        int rr = c + 1 < n - 1 - c ? c + 1 : n - 1 - c;  // fit_fwd: r <= c + 1 and r <= n - 1 - c
-        rr = rr > R ? R : (rr < 1 ? 1 : rr);
-        float d[R];
-#pragma unroll
-        for (int r = 1; r <= R; ++r) {
-            const bool use = live && r <= rr;
-            const int hi = opaque(use ? r : 0), lo = opaque(use ? r - 1 : 0);
-            d[r - 1] = f[id + hi * st] - f[id - lo * st];
-        }
-        float out = 0.0f;
-#pragma unroll
-        for (int q = 1; q <= R; ++q) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int r = 1; r <= q; ++r) acc += stencil_c(q, r) * d[r - 1];
-            out = rr == q ? acc : out;
-        }
-        return out;
-    }
 }
 
 __device__ __forceinline__ int fit_bwd_sel(int c, int n, int R)
@@ -220,6 +155,75 @@ __device__ __forceinline__ int fit_fwd_sel(int c, int n, int R)
 {
     int r = c + 1 < n - 1 - c ? c + 1 : n - 1 - c;
     return r > R ? R : (r < 1 ? 1 : r);
+}
+
+// Branch-free forms of dfwd / dbwd for the general (CPML shell) path of the
+// 3D sweeps, in two halves: the taps (loads only) and their combination
+// (dfwd_comb serves both), so that a caller issues the loads of all its
+// differences before any of their arithmetic. With live, each tap the
+// fitted stencil uses is loaded from the address dfwd / dbwd load, and the
+// other taps read the cell itself; without live every tap reads the cell
+// itself (the caller does not use the result). Every candidate radius's
+// sum is formed in the reference's order and the fitted one selected, so
+// the same bits come out, while a wavefront waits on memory once instead of
+// once per branch.
+template <int ORDER>
+struct FwdTaps {
+    static constexpr int R = Radius<ORDER>::value;
+    float hi[R], lo[R];
+    int rr;
+};
+
+template <int ORDER>
+__device__ __forceinline__ void dfwd_taps(FwdTaps<ORDER>& t, const float* __restrict__ f, long long id, long long st,
+                                          int c, int n, bool live)
+{
+    constexpr int R = Radius<ORDER>::value;
+    int rr = c + 1 < n - 1 - c ? c + 1 : n - 1 - c;
+    t.rr = rr > R ? R : (rr < 1 ? 1 : rr);
+#pragma unroll
+    for (int r = 1; r <= R; ++r) {
+        const bool use = live && (ORDER <= 2 || r <= t.rr);
+        const int hi = opaque(use ? r : 0), lo = opaque(use ? r - 1 : 0);
+        t.hi[r - 1] = f[id + hi * st];
+        t.lo[r - 1] = f[id - lo * st];
+    }
+}
+
+template <int ORDER>
+__device__ __forceinline__ float dfwd_comb(const FwdTaps<ORDER>& t)
+{
+    if constexpr (ORDER <= 2) {
+        return t.hi[0] - t.lo[0];
+    } else {
+        constexpr int R = Radius<ORDER>::value;
+        float out = 0.0f;
+#pragma unroll
+        for (int q = 1; q <= R; ++q) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int r = 1; r <= q; ++r) acc += stencil_c(q, r) * (t.hi[r - 1] - t.lo[r - 1]);
+            out = t.rr == q ? acc : out;
+        }
+        return out;
+    }
+}
+
+// The taps of dbwd.
+template <int ORDER>
+__device__ __forceinline__ void dbwd_taps(FwdTaps<ORDER>& t, const float* __restrict__ f, long long id, long long st,
+                                          int c, int n, bool live)
+{
+    constexpr int R = Radius<ORDER>::value;
+    int rr = c < n - c ? c : n - c;
+    t.rr = rr > R ? R : (rr < 1 ? 1 : rr);
+#pragma unroll
+    for (int r = 1; r <= R; ++r) {
+        const bool use = live && (ORDER <= 2 || r <= t.rr);
+        const int hi = opaque(use ? r - 1 : 0), lo = opaque(use ? r : 0);
+        t.hi[r - 1] = f[id + hi * st];
+        t.lo[r - 1] = f[id - lo * st];
+    }
 }
 
 // The unclamped differences (the full-radius branch of dbwd / dfwd), for

@@ -116,10 +116,18 @@ __device__ __forceinline__ void fwd_h_cell(const Grid& g, const float* __restric
         }
     }
     float dv[6];
+    {
+        // every tap of the six differences is loaded before any arithmetic
+        // (a difference the cell does not need reads the cell itself and is
+        // not used)
+        FwdTaps<ORDER> tp[6];
 #pragma unroll
-    for (int d = 0; d < 6; ++d) {
-        // a difference the cell does not need reads the cell itself and is not used
-        dv[d] = dfwd_sel<ORDER>(E.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]], g.N[kDiffAxis[d]], need[d]);
+        for (int d = 0; d < 6; ++d)
+            dfwd_taps<ORDER>(tp[d], E.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]], g.N[kDiffAxis[d]],
+                             need[d]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) dv[d] = dfwd_comb<ORDER>(tp[d]);
     }
     // selects rather than branches, so that every load above is issued at once
     const float n0 = h[0] - uy * dv[0] + uz * dv[1];  // dEz/dy, dEy/dz
@@ -288,11 +296,18 @@ __device__ __forceinline__ void fwd_e_cell(const Grid& g, const float* __restric
             }
         }
         float dv[6];
+        {
+            // every tap of the six differences is loaded before any arithmetic
+            // (a difference the cell does not need reads the cell itself and
+            // is not used)
+            FwdTaps<ORDER> tp[6];
 #pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            // a difference the cell does not need reads the cell itself and is not used
-            dv[d] = dbwd_sel<ORDER>(Hf.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]], g.N[kDiffAxis[d]],
-                                    need[d]);
+            for (int d = 0; d < 6; ++d)
+                dbwd_taps<ORDER>(tp[d], Hf.c[kDiffComp[d]], n, st[kDiffAxis[d]], c[kDiffAxis[d]], g.N[kDiffAxis[d]],
+                                 need[d]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int d = 0; d < 6; ++d) dv[d] = dfwd_comb<ORDER>(tp[d]);
         }
         // selects rather than branches, so that every load above is issued at once
         const float n0 = u0 * e[0] + uy * dv[0] - uz * dv[1];  // dHz/dy, dHy/dz
