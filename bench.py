@@ -175,11 +175,11 @@ def kernel_bytes(kernel: str, w, args) -> float:
 MEASURED_TRAFFIC = {
     ("c2", "tm_step_adj", "float32", "float64", 20): (121.9e6, "profiles/c2_fwi_2d_o4_tm_step_kernels.md"),
     ("c2", "tm_step_fwd", "float32", "float64", 20): (90.4e6, "profiles/c2_fwi_2d_o4_tm_step_kernels.md"),
-    ("c3", "fwd_e", "float32", "float64", 1): (1662.8e6, "profiles/c3_layered_3d_o4_kernels.md"),
-    ("c3", "fwd_h", "float32", "float64", 1): (1412.5e6, "profiles/c3_layered_3d_o4_kernels.md"),
-    ("c3", "adj_e_w", "float32", "float64", 1): (2204.4e6, "profiles/c3_layered_3d_o4_kernels.md"),
-    ("c3", "adj_h", "float32", "float64", 1): (1519.8e6, "profiles/c3_layered_3d_o4_kernels.md"),
-    ("c3", "adj_weights_e", "float32", "float64", 1): (1059.2e6, "profiles/c3_layered_3d_o4_kernels.md"),
+    ("c3", "fwd_e", "float32", "float64", 1): (1656.4e6, "profiles/c3_layered_3d_o4_kernels.md"),
+    ("c3", "fwd_h", "float32", "float64", 1): (1383.7e6, "profiles/c3_layered_3d_o4_kernels.md"),
+    ("c3", "adj_e_w", "float32", "float64", 1): (2204.0e6, "profiles/c3_layered_3d_o4_kernels.md"),
+    ("c3", "adj_h", "float32", "float64", 1): (1520.4e6, "profiles/c3_layered_3d_o4_kernels.md"),
+    ("c3", "adj_weights_e", "float32", "float64", 1): (1058.7e6, "profiles/c3_layered_3d_o4_kernels.md"),
 }
 
 
