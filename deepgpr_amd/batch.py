@@ -37,6 +37,7 @@ from .native.bridge import build_solve
 from .native.loader import require_library
 from .solver.autograd import forward_native_args
 from .solver.modeling import prepare_compute
+from .utils.devices import resolve_device
 from .utils.exceptions import NativeLibraryError
 
 
@@ -84,7 +85,7 @@ class BatchSolver:
     ``step`` reports once the result is complete and ``result`` reads."""
 
     def __init__(self, device="cuda:0", stream: Optional[torch.cuda.Stream] = None, lookahead: int = 2):
-        self.device = torch.device(device)
+        self.device = resolve_device(device)
         self.lookahead = int(lookahead)  # step() calls the host may run ahead of the device
         self._marks: collections.deque = collections.deque()
         self._lib = require_library(self.device)

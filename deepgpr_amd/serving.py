@@ -38,6 +38,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 
 import torch
 
+from .utils.devices import resolve_device
 from .solver.memory import estimate_compute_memory
 
 
@@ -159,7 +160,7 @@ class SolveServer:
                  batch_policy: str = "alone"):
         if batch_policy not in ("alone", "always"):
             raise ValueError(f"batch_policy must be 'alone' or 'always', not {batch_policy!r}")
-        self.device = torch.device(device)
+        self.device = resolve_device(device)
         if hbm_budget is None:
             hbm_budget = int(hbm_fraction * torch.cuda.mem_get_info(self.device)[0])
         self.budget = int(hbm_budget)
@@ -405,7 +406,7 @@ def run_stream(requests: Sequence[SolveRequest], device="cuda:0", slots: int = 8
 
 def run_sequential(requests: Sequence[SolveRequest], device="cuda:0") -> Dict:
     """The same requests one at a time through plain compute(), back to back."""
-    dev = torch.device(device)
+    dev = resolve_device(device)
     results: Dict[int, SolveResult] = {}
     t0 = time.perf_counter()
     for req in requests:

@@ -19,6 +19,7 @@ import torch
 from ..config.constants import CPML_FACE_NAMES, TM2D_INACTIVE_PHI_INDICES
 from ..native.bridge import invoke
 from ..native.loader import library_supports, require_library
+from ..utils.devices import resolve_device
 from ..utils.exceptions import NativeLibraryError, NonFiniteTensorError
 from .storage import (
     WavefieldStorageConfig, decompress_wavefield_history, encode_storage_type, history_spatial_shape,
@@ -119,14 +120,6 @@ def _check_capabilities(lib: Any, config: SolverConfig) -> None:
             raise NativeLibraryError(f"The loaded HIP library lacks {cap}; rebuild it.")
 
 
-def _indexed(device) -> torch.device:
-    """`device` with its index filled in (a bare "cuda" is the current device)."""
-    device = torch.device(device)
-    if device.type == "cuda" and device.index is None:
-        return torch.device("cuda", torch.cuda.current_device())
-    return device
-
-
 class HistoryPool:
     """One device buffer that solves run one after another take their
     histories from, instead of allocating them each time.
@@ -153,7 +146,7 @@ class HistoryPool:
     ALIGN = 256
 
     def __init__(self, device):
-        self.device = _indexed(device)
+        self.device = resolve_device(device)
         self.buffer: Optional[torch.Tensor] = None
         self.keep = True
         self._mem = None  # the private allocator pool of the current buffer
@@ -232,7 +225,7 @@ def _allocate_histories(config: SolverConfig, needs_model_gradient: bool, recons
     elif needs_model_gradient:
         specs.append(frame)
     pool = _HISTORY_POOL.get()
-    if pool is not None and pool.device == _indexed(dev):
+    if pool is not None and pool.device == resolve_device(dev):
         got = pool.take(specs)
     else:
         got = [torch.empty(sh, device=dev, dtype=dt) for sh, dt in specs]

@@ -25,6 +25,7 @@ from ..config.defaults import (
 from ..native.loader import library_supports, require_library
 from ..preprocessing.grid import normalize_grid_spacing
 from ..preprocessing.model_setup import PreparedModel, initialization, shift_locations
+from ..utils.devices import resolve_device
 from ..utils.logger import get_logger
 from .autograd import TM2D_INACTIVE_STATE_INDICES, DeepGPR, SolverConfig, SolverTensors
 from .fields import create_or_separate
@@ -221,7 +222,7 @@ def prepare_compute(device, dx=None, dt=None, source_amplitudes=None, source_loc
     if gradient_accumulator not in GRADIENT_ACCUMULATORS:
         raise ValueError("gradient_accumulator must be 'float32' or 'float64'.")
 
-    device = torch.device(device)
+    device = resolve_device(device)
     if fdtd_order not in SUPPORTED_FDTD_ORDERS:
         raise ValueError("fdtd_order must be one of 2, 4, or 8.")
     if mode not in SUPPORTED_GRADIENT_MODES:

@@ -30,6 +30,7 @@ from .autograd import HistoryPool, history_pool
 from .fields import checkpoint_initial_field
 from .memory import estimate_compute_memory
 from .modeling import compute
+from ..utils.devices import resolve_device
 
 _STORAGE_NAMES = {torch.float32: "float32", torch.float16: "float16", torch.bfloat16: "bfloat16"}
 
@@ -102,7 +103,7 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
                          "is not accepted")
     if model_gradient_sampling_interval == "auto":
         raise ValueError("segmented_compute needs an explicit model_gradient_sampling_interval")
-    device = torch.device(device)
+    device = resolve_device(device)
     nt = int(source_amplitudes.shape[1])
     if nt < 1:
         raise ValueError("segmented_compute needs at least one time step")

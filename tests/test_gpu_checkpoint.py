@@ -89,7 +89,7 @@ def test_checkpoint_segments_match_the_full_solve(cuda_device, shape, pml, reent
     ((8, 9), (2, 3, 1, 2), 12, None), ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, None),
     ((30, 26, 34), (4, 5, 3, 4, 5, 3), None, None),
     ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, "bfloat16"), ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, "int8"),
-    ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, "sampled"),
+    ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, "sampled"), ((30, 26, 34), (4, 5, 3, 4, 5, 3), 12, "bare_cuda"),
 ])
 def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segment_steps, history, monkeypatch):
     """The library's segment driver: traces and final states bit for bit, and
@@ -104,7 +104,8 @@ def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segme
     base, sigma_base, args = _problem(shape, pml, cuda_device)
     args.update({None: {}, "bfloat16": {"wavefield_storage_dtype": torch.bfloat16},
                  "int8": {"wavefield_compression": "int8"},
-                 "sampled": {"model_gradient_sampling_interval": 2}}[history])
+                 "sampled": {"model_gradient_sampling_interval": 2}, "bare_cuda": {}}[history])
+    device = "cuda" if history == "bare_cuda" else cuda_device  # the pool matches devices by index
     ref_model = base.clone().requires_grad_()
     ref_sigma = sigma_base.clone().requires_grad_()
     full = dg.compute(eps_r=ref_model, sigma=ref_sigma, **args)
@@ -128,10 +129,15 @@ def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segme
     class Recorded(segments.HistoryPool):
         def __init__(self, device):
             super().__init__(device)
+            self.takes = 0
             pools.append(self)
 
+        def take(self, specs):
+            self.takes += 1
+            return super().take(specs)
+
     monkeypatch.setattr(segments, "HistoryPool", Recorded)
-    out = dg.segmented_compute(cuda_device, eps_r=model, sigma=sigma, segment_steps=segment_steps,
+    out = dg.segmented_compute(device, eps_r=model, sigma=sigma, segment_steps=segment_steps,
                                memory_budget=budget, **kw)
     assert torch.equal(out[-1], full[-1])
     for a, b in zip((*out[1], *out[2], *out[3]), (*full[1], *full[2], *full[3])):
@@ -139,8 +145,14 @@ def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segme
     out[-1].square().sum().backward()
     for a, b in ((model.grad, ref_model.grad), (sigma.grad, ref_sigma.grad)):
         torch.testing.assert_close(a, b, rtol=2.0e-5, atol=max(1.0e-5, 1.0e-7 * float(b.abs().max())))
-    # the re-runs took their histories from the pool, which let go of them after the backward
+    # the last segment's run and every re-run took their histories from the
+    # pool, one take per segment, and the pool let go of them after the backward
     assert len(pools) == 1 and pools[0].buffer is None and not pools[0].keep
+    if segment_steps is None:
+        assert pools[0].takes >= 2
+    else:
+        S = kw.get("model_gradient_sampling_interval", 1)
+        assert pools[0].takes == len(segments.segment_bounds(args["source_amplitudes"].shape[1], segment_steps, S))
 
 
 def test_segmented_compute_on_the_c3_grid(cuda_device):
