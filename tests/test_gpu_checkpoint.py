@@ -190,3 +190,27 @@ def test_segmented_compute_on_the_c3_grid(cuda_device):
         assert torch.equal(a, b)
     for a, b in ((seg[2], eps2.grad), (seg[3], sig2.grad)):
         torch.testing.assert_close(a, b, rtol=2.0e-5, atol=max(1.0e-5, 1.0e-7 * float(b.abs().max())))
+
+
+def test_segmented_compute_forward_only(cuda_device, monkeypatch):
+    """Without a gradient to take, every segment runs without a history (the
+    pool is never used), and traces and final states equal one compute()."""
+    import deepgpr_amd as dg
+    from deepgpr_amd.solver import segments
+    print("[gpu checkpoint] segmented_compute forward only", flush=True)
+    base, sigma_base, args = _problem((30, 26, 34), (4, 5, 3, 4, 5, 3), cuda_device)
+    full = dg.compute(eps_r=base, sigma=sigma_base, **args)
+    takes = []
+
+    class Recorded(segments.HistoryPool):
+        def take(self, specs):
+            takes.append(specs)
+            return super().take(specs)
+
+    monkeypatch.setattr(segments, "HistoryPool", Recorded)
+    kw = {k: v for k, v in args.items() if k != "device"}
+    out = dg.segmented_compute(cuda_device, eps_r=base, sigma=sigma_base, segment_steps=12, **kw)
+    assert not takes
+    assert out[0] is None and torch.equal(out[-1], full[-1])
+    for a, b in zip((*out[1], *out[2], *out[3]), (*full[1], *full[2], *full[3])):
+        assert torch.equal(a, b)
