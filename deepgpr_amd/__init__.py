@@ -8,9 +8,12 @@ hand-written HIP kernels for gfx950 in ``libdeepgpr_amd.so`` (C ABI in
 """
 from __future__ import annotations
 
-from . import config, utils  # noqa: F401
+from . import config, utils, wavelet  # noqa: F401
 from .config.constants import SPEED_OF_LIGHT as c  # noqa: F401
-from .native.loader import get_library, library_path  # noqa: F401
+from .native.loader import (  # noqa: F401
+    clear_cuda_resource_cache, get_deepgpr_lib, get_deepgpr_library_path, get_library, library_path,
+    set_library_fdtd_order,
+)
 from .preprocessing.grid import pmlthick_revert  # noqa: F401
 from .preprocessing.model_setup import initialization  # noqa: F401
 from .preprocessing.stability import check_cfl, max_stable_time_step  # noqa: F401
@@ -25,6 +28,7 @@ from .solver.pml import (  # noqa: F401
 from .solver.sampling import recommended_sampling_interval, source_max_frequency  # noqa: F401
 from .solver.segments import plan_segments, segmented_compute  # noqa: F401
 from .solver.storage import decompress_wavefield_history  # noqa: F401
+from .utils.logger import configure_logging, get_logger  # noqa: F401
 from .utils.exceptions import (  # noqa: F401
     CFLConditionError, DeepGPRError, NativeLibraryError, NativeLibraryNotFoundError, NonFiniteTensorError,
 )

@@ -81,3 +81,60 @@ def require_library(device) -> ctypes.CDLL:
 def library_supports(lib, capability: str) -> bool:
     probe = getattr(lib, capability, None)
     return probe is not None and int(probe()) == 1
+
+
+# The reference loader's public helpers (src/DeepGPR/native/loader.py:222-293,
+# exported at src/DeepGPR/__init__.py:36-41), with the same names, arguments
+# and errors: an unknown device kind is a ValueError; the CPU kind, which the
+# reference serves with its C backend, has no library here and raises
+# NativeLibraryError, as compute() does for a CPU device.
+
+def get_deepgpr_lib(device) -> ctypes.CDLL:
+    """The library serving `device` ("cuda", "cuda:N", a torch.device)."""
+    import torch
+
+    kind = torch.device(device).type
+    if kind not in ("cpu", "cuda"):
+        raise ValueError(f"Unsupported DeepGPR device: {device}")
+    return require_library(device)
+
+
+def get_deepgpr_library_path(device) -> str:
+    """Filesystem path of the library serving `device`."""
+    get_deepgpr_lib(device)
+    return library_path()
+
+
+def clear_cuda_resource_cache(device=None) -> None:
+    """Release the library's idle per-device resources (None: the current
+    device; an int: that device index). Histories stay in HBM here, so the
+    library holds no offload buffers and the call releases nothing; it keeps
+    the reference's device checks and error reporting."""
+    import torch
+
+    if isinstance(device, int):
+        target = torch.device("cuda", device)
+    else:
+        target = torch.device("cuda" if device is None else device)
+    if target.type != "cuda":
+        raise ValueError("clear_cuda_resource_cache requires a CUDA device")
+    lib = get_deepgpr_lib(target)
+    with torch.cuda.device(target):
+        lib.deepgpr_clear_last_error()
+        lib.deepgpr_clear_cuda_resource_cache()
+        message = lib.deepgpr_last_error()
+    if message:
+        if isinstance(message, bytes):
+            message = message.decode("utf-8", "replace")
+        raise NativeLibraryError(f"deepgpr_clear_cuda_resource_cache failed: {message}")
+
+
+def set_library_fdtd_order(lib, fdtd_order: int) -> None:
+    """Select the FDTD order of the ABI-6 `forward` / `backward` entry points
+    (process-global state in the library; the dgx_* entry points take the
+    order in their descriptor and ignore it)."""
+    from ..config.constants import SUPPORTED_FDTD_ORDERS
+
+    if fdtd_order not in SUPPORTED_FDTD_ORDERS:
+        raise ValueError("fdtd_order must be one of 2, 4, or 8.")
+    lib.set_fdtd_order(int(fdtd_order))

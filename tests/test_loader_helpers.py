@@ -1,0 +1,75 @@
+"""The reference's top-level loader and logging helpers
+(src/DeepGPR/__init__.py:36-45, native/loader.py:222-293, utils/logger.py:36-64,
+wavelet.py)
+under the same names, with the same argument checks and errors."""
+import io
+import logging
+
+import pytest
+
+import deepgpr_amd as dg
+from deepgpr_amd.utils.exceptions import NativeLibraryError
+
+NAMES = ("clear_cuda_resource_cache", "get_deepgpr_lib", "get_deepgpr_library_path", "set_library_fdtd_order",
+         "configure_logging", "get_logger")
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_exported_at_top_level(name):
+    assert callable(getattr(dg, name))
+
+
+def test_library_for_device_kinds():
+    from deepgpr_amd.native.loader import library_path
+    assert dg.get_deepgpr_library_path("cuda") == library_path()
+    assert dg.get_deepgpr_lib("cuda:0") is dg.get_library()
+    with pytest.raises(NativeLibraryError):  # no CPU backend in this build
+        dg.get_deepgpr_lib("cpu")
+    with pytest.raises(ValueError):
+        dg.get_deepgpr_lib("meta")
+
+
+def test_set_library_fdtd_order_checks_the_order():
+    lib = dg.get_deepgpr_lib("cuda")
+    for bad in (0, 3, 6, 16):
+        with pytest.raises(ValueError):
+            dg.set_library_fdtd_order(lib, bad)
+    for order in (4, 8, 2):  # host-side state only; leaves the default order 2 set
+        dg.set_library_fdtd_order(lib, order)
+
+
+def test_clear_cuda_resource_cache_refuses_a_cpu_device():
+    with pytest.raises(ValueError):
+        dg.clear_cuda_resource_cache("cpu")
+
+
+def test_configure_logging_replaces_its_handler():
+    buf = io.StringIO()
+    logger = dg.configure_logging("DEBUG", "%(name)s|%(message)s", stream=buf)
+    try:
+        dg.configure_logging("DEBUG", "%(name)s|%(message)s", stream=buf)
+        ours = [h for h in logger.handlers if isinstance(h, logging.StreamHandler)
+                and not isinstance(h, logging.NullHandler)]
+        assert len(ours) == 1
+        dg.get_logger("solver").debug("one line")
+        assert buf.getvalue() == "DeepGPR.solver|one line\n"
+        assert not set(ours) & set(logging.getLogger().handlers)  # the root logger is left alone
+    finally:
+        for h in list(logger.handlers):
+            if getattr(h, "_deepgpr_amd_console", False):
+                logger.removeHandler(h)
+        logger.setLevel(logging.NOTSET)
+
+
+@pytest.mark.gpu
+def test_clear_cuda_resource_cache_on_the_device(cuda_device):
+    print("[gpu loader] clear_cuda_resource_cache", flush=True)
+    dg.clear_cuda_resource_cache()
+    dg.clear_cuda_resource_cache(0)
+    dg.clear_cuda_resource_cache(cuda_device)
+
+
+def test_wavelet_namespace_of_the_quick_start():
+    assert dg.wavelet.ricker is dg.ricker
+    w = dg.wavelet.ricker(2e8, 2000, 3e-11, 5e-9)
+    assert w.shape == (2000,) and float(w.abs().max()) == pytest.approx(1.0, abs=1e-3)
