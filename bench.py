@@ -186,15 +186,15 @@ MEASURED_TRAFFIC = {
 def _shots_per_call(w, args, free_bytes: int) -> int:
     """Shots per compute() call: all of them when the whole solve fits in 85 %
     of free HBM, else as many as fit (a history larger than HBM runs shot by shot)."""
-    from deepgpr_amd.solver.memory import estimate_compute_memory
+    from deepgpr_amd.solver.memory import solve_footprint
 
     shape = [n + 2 * w.pml for n in w.physical_shape] + ([1] if len(w.physical_shape) == 2 else [])
     comps, hist_b, frames = _history_terms(w, args)
-    one = estimate_compute_memory(*shape, w.nt, 1, [w.pml] * 6, mode=w.mode, sampling_interval=w.sampling_interval,
-                                  storage="float32" if args.storage == "int8" else args.storage,
-                                  compression="int8" if args.storage == "int8" else "none",
-                                  reconstruct_rhs=frames == 1, fp64_accumulators=args.accumulator == "float64",
-                                  nrx=w.receiver_location.shape[1])["total"]
+    one = solve_footprint(*shape, w.nt, 1, [w.pml] * 6, mode=w.mode, sampling_interval=w.sampling_interval,
+                          storage="float32" if args.storage == "int8" else args.storage,
+                          compression="int8" if args.storage == "int8" else "none",
+                          reconstruct_rhs=frames == 1, fp64_accumulators=args.accumulator == "float64",
+                          nrx=w.receiver_location.shape[1])["total"]
     shots = w.source_location.shape[0]
     fit = max(1, int(0.85 * free_bytes // max(one, 1)))
     return min(shots, fit)

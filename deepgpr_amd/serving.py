@@ -10,8 +10,8 @@ take a stream otherwise (``batch_policy``). Other requests (3D, INT8
 histories, ...) each run on a HIP stream of their own, so their kernels
 overlap on the device. A request's state (fields, CPML
 auxiliaries, histories) stays resident in HBM for the life of the request.
-Admission prices each request with ``estimate_compute_memory`` (the byte
-model of ``src/DeepGPR/solver/memory.py:30-268``); a request that does not fit
+Admission prices each request with ``solve_footprint`` (a byte model in the
+manner of ``src/DeepGPR/solver/memory.py:30-246``); a request that does not fit
 waits, and smaller requests behind it may go first within a bounded window.
 A result does not depend on what else is in flight: the kernels are
 deterministic and every request owns its arrays.
@@ -39,7 +39,7 @@ from typing import Callable, Dict, List, Optional, Sequence
 import torch
 
 from .utils.devices import resolve_device
-from .solver.memory import estimate_compute_memory
+from .solver.memory import solve_footprint
 
 
 @dataclass
@@ -102,11 +102,11 @@ def footprint_bytes(req: SolveRequest) -> int:
     opt = req.options
     storage = opt.get("wavefield_storage_dtype", torch.float32)
     name = {torch.float32: "float32", torch.float16: "float16", torch.bfloat16: "bfloat16"}[storage]
-    est = estimate_compute_memory(*shape, req.nt, int(req.source_location.shape[0]), [req.pmlthick] * 6,
-                                  mode=opt.get("mode", 2), sampling_interval=opt.get("model_gradient_sampling_interval", 1),
-                                  storage=name, compression=opt.get("wavefield_compression", "none"),
-                                  reconstruct_rhs=opt.get("wavefield_rhs_history", "auto") != "stored" and name == "float32",
-                                  gradient=req.gradient, nrx=int(req.receiver_location.shape[1]))
+    est = solve_footprint(*shape, req.nt, int(req.source_location.shape[0]), [req.pmlthick] * 6,
+                          mode=opt.get("mode", 2), sampling_interval=opt.get("model_gradient_sampling_interval", 1),
+                          storage=name, compression=opt.get("wavefield_compression", "none"),
+                          reconstruct_rhs=opt.get("wavefield_rhs_history", "auto") != "stored" and name == "float32",
+                          gradient=req.gradient, nrx=int(req.receiver_location.shape[1]))
     total = est["total"] - (0 if req.gradient else est["history"])  # forward-only solves keep no history
     return int(total * 1.1)
 

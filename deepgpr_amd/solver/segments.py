@@ -28,7 +28,7 @@ from ..config.defaults import (DEFAULT_FDTD_ORDER, DEFAULT_GRADIENT_ACCUMULATOR,
 from ..preprocessing.model_setup import initialization
 from .autograd import HistoryPool, history_pool
 from .fields import checkpoint_initial_field
-from .memory import estimate_compute_memory
+from .memory import solve_footprint
 from .modeling import compute
 from ..utils.devices import resolve_device
 
@@ -39,16 +39,15 @@ def plan_segments(nt: int, budget_bytes: int, grid, nstep: int, pml, *, mode: in
                   storage: str = "float32", compression: str = "none", block_size=(4, 4, 4),
                   reconstruct_rhs: bool = False, fp64_accumulators: bool = True, nrx: int = 1) -> int:
     """Steps per segment: the longest multiple of `sampling_interval` whose
-    solve (estimate_compute_memory over the extended grid `grid` = (nx, ny,
+    solve (solve_footprint over the extended grid `grid` = (nx, ny,
     nz)) fits `budget_bytes`, at most nt. Raises ValueError when not even one
     sampling interval fits."""
     S = max(int(sampling_interval), 1)
 
     def need(steps: int) -> int:
-        return estimate_compute_memory(*grid, steps, nstep, pml, mode=mode, sampling_interval=S, storage=storage,
-                                       compression=compression, block_size=block_size,
-                                       reconstruct_rhs=reconstruct_rhs, fp64_accumulators=fp64_accumulators,
-                                       nrx=nrx)["total"]
+        return solve_footprint(*grid, steps, nstep, pml, mode=mode, sampling_interval=S, storage=storage,
+                               compression=compression, block_size=block_size, reconstruct_rhs=reconstruct_rhs,
+                               fp64_accumulators=fp64_accumulators, nrx=nrx)["total"]
 
     if need(S) > budget_bytes:
         raise ValueError(f"segmented_compute: one segment of {S} steps needs {need(S)} bytes, "
@@ -88,7 +87,7 @@ def segmented_compute(device, dx=None, dt=None, source_amplitudes=None, source_l
     sampling interval); by default the longest segment whose solve fits
     `memory_budget` bytes (default: 60 % of the device's free memory,
     counting what torch's allocator holds unused as free), from
-    estimate_compute_memory. Other keywords go to compute() unchanged (E, H
+    solve_footprint. Other keywords go to compute() unchanged (E, H
     and PML initial states are not accepted: the run starts from zero states,
     as checkpoint_initial_field gives them; nor is save_forward_wavefield_path).
 

@@ -116,11 +116,12 @@ def test_segmented_compute_matches_the_full_solve(cuda_device, shape, pml, segme
     kw = {k: v for k, v in args.items() if k != "device"}
     budget = None
     if segment_steps is None:
+        from deepgpr_amd.solver.memory import solve_footprint
         # the driver's own estimate (E-only FP32 history at S 1, FP32 sums) of a third of the steps
         n = [s + p0 + p1 for s, p0, p1 in zip(shape, pml[0::2], pml[1::2])]
         nt = args["source_amplitudes"].shape[1]
         opts = dict(mode=args["mode"], reconstruct_rhs=True, fp64_accumulators=False, nrx=2)
-        budget = dg.estimate_compute_memory(*n, nt // 3, 2, list(pml), **opts)["total"]
+        budget = solve_footprint(*n, nt // 3, 2, list(pml), **opts)["total"]
         planned = dg.plan_segments(nt, budget, n, 2, list(pml), **opts)
         assert planned < nt, planned  # several segments
     from deepgpr_amd.solver import segments

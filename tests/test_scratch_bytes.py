@@ -1,6 +1,6 @@
 """dgx_scratch_bytes (host logic only, no GPU): the device memory a solve
 needs beyond the caller's arrays, and the matching term of
-estimate_compute_memory used for admission."""
+solve_footprint used for admission."""
 import ctypes
 import os
 
@@ -9,7 +9,7 @@ import torch
 
 from deepgpr_amd.native.bridge import build_solve
 from deepgpr_amd.native.loader import get_library
-from deepgpr_amd.solver.memory import estimate_compute_memory
+from deepgpr_amd.solver.memory import solve_footprint
 
 _T = torch.zeros(8)  # stands in for every array: only the sizes are read
 
@@ -47,5 +47,5 @@ def test_2d_tm_adjoint_needs_no_weight_arrays():
 
 @pytest.mark.skipif(SPLIT, reason="the estimate prices the default (fused) adjoint")
 def test_estimate_matches_the_library():
-    est = estimate_compute_memory(29, 30, 31, 10, 2, [0] * 6, mode=3)
+    est = solve_footprint(29, 30, 31, 10, 2, [0] * 6, mode=3)
     assert est.get("adjoint_weights", 0) == _scratch(30, 31, 32, 2)

@@ -3,7 +3,8 @@ multiple of the sampling interval whose solve fits the memory budget."""
 import pytest
 import torch
 
-from deepgpr_amd import estimate_compute_memory, plan_segments, segmented_compute
+from deepgpr_amd import plan_segments, segmented_compute
+from deepgpr_amd.solver.memory import solve_footprint
 from deepgpr_amd.solver.segments import segment_bounds
 from deepgpr_amd.solver.autograd import HistoryPool, history_pool
 
@@ -11,8 +12,8 @@ C3 = dict(grid=(288, 288, 224), nstep=4, pml=[16] * 6)
 
 
 def _need(steps, S=4):
-    return estimate_compute_memory(*C3["grid"], steps, C3["nstep"], C3["pml"], mode=3, sampling_interval=S,
-                                   nrx=256)["total"]
+    return solve_footprint(*C3["grid"], steps, C3["nstep"], C3["pml"], mode=3, sampling_interval=S,
+                           nrx=256)["total"]
 
 
 def test_c3_four_shots_fp32_history_needs_segments():
