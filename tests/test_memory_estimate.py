@@ -116,3 +116,15 @@ def test_cpu_device_has_no_backend():
         dg.estimate_compute_memory(**_common(device=torch.device("cpu")))
     assert math.isclose(dg.estimate_compute_memory(**_common(device="cuda:0"))["estimated_device_peak"],
                         dg.estimate_compute_memory(**_common())["estimated_device_peak"])
+
+
+def test_estimate_scales_with_physical_cells():
+    # restates tests/test_history_region.py:140-153 (on a HIP device: this
+    # build has no CPU backend)
+    common = dict(device=CUDA, nx=30, ny=26, nz=1, nt=50, nstep=2, nsr=1, nrx=3, source_waveforms=1,
+                  pml=[3, 5, 4, 2, 0, 0], mode=2, sampling_interval=1, storage_dtype=torch.float32,
+                  use_async_offload=False, er_requires_grad=False, se_requires_grad=False)
+    extended = dg.estimate_compute_memory(**common)
+    physical = dg.estimate_compute_memory(history_region="physical", **common)
+    assert physical["history_shape"] == (22, 20, 1)
+    assert physical["saved_gradient_wavefields"] * 30 * 26 == extended["saved_gradient_wavefields"] * 22 * 20
