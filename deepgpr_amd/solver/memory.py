@@ -157,19 +157,68 @@ def solve_footprint(nx: int, ny: int, nz: int, nt: int, nstep: int, pml, *, mode
     return out
 
 
-def format_compute_preview(config) -> str:
-    est = solve_footprint(
-        config.nx, config.ny, config.nz, config.nt, config.nstep, config.pml, mode=config.mode,
-        sampling_interval=config.sampling_interval, storage=str(config.storage.dtype).replace("torch.", ""),
-        compression=config.storage.compression, block_size=config.storage.block_size or (4, 4, 4),
-        reconstruct_rhs=config.reconstruct_rhs, fp64_accumulators=config.grad_accumulator == "float64",
-        nrx=config.nrx)
+# report rows: (label, estimate key), the rows of the reference's report
+# (src/DeepGPR/solver/memory.py:249-264) with this build's scratch terms
+_REPORT_ROWS = (
+    ("model and padded materials", "model_and_padded_materials"),
+    ("electric and magnetic fields", "electric_and_magnetic_fields"),
+    ("CPML auxiliary fields", "cpml_auxiliary_fields"),
+    ("CPML coefficients", "cpml_coefficients"),
+    ("source and acquisition locations", "source_and_locations"),
+    ("saved E_saved and R_saved wavefields", "saved_gradient_wavefields"),
+    ("  of which final E frame (E-only history)", "final_e_frame"),
+    ("FDTD update coefficients", "fdtd_update_coefficients"),
+    ("receiver data", "receiver_data"),
+    ("material gradients (with accumulators)", "material_gradients"),
+    ("adjoint fields and CPML", "adjoint_fields_and_cpml"),
+    ("receiver adjoint", "receiver_adjoint"),
+    ("3D adjoint channel weights", "adjoint_stencil_weights"),
+)
+
+
+def _mib(nbytes: int) -> str:
+    return f"{nbytes / 2**20:.2f} MiB"
+
+
+def format_compute_preview(config, *, source_waveforms: int, er_requires_grad: bool, se_requires_grad: bool,
+                           save_directory=None) -> str:
+    """The print_parameters report of compute() (in the manner of
+    src/DeepGPR/solver/memory.py:267-456): the resolved options, then the
+    device-memory estimate of estimate_compute_memory by category."""
+    st = config.storage
+    est = estimate_compute_memory(
+        device=config.device, nx=config.nx, ny=config.ny, nz=config.nz, nt=config.nt, nstep=config.nstep,
+        nsr=config.nsr, nrx=config.nrx, source_waveforms=source_waveforms, pml=config.pml, mode=config.mode,
+        sampling_interval=config.sampling_interval, storage_dtype=st.dtype,
+        use_async_offload=config.use_async_offload, er_requires_grad=er_requires_grad,
+        se_requires_grad=se_requires_grad, save_wavefield_history=config.save_wavefield_history,
+        wavefield_compression=st.compression,
+        compression_block_size=st.block_size if st.compression == "int8" else None,
+        reconstruct_rhs=config.reconstruct_rhs, history_region=config.history_region,
+        gradient_accumulator=config.grad_accumulator)
+    dx, dy, dz = config.grid_spacing
     lines = [
-        "deepgpr_amd compute",
-        f"  device={config.device} order={config.fdtd_order} mode={config.mode}",
-        f"  extended grid=({config.nx}, {config.ny}, {config.nz}) nt={config.nt} shots={config.nstep}",
-        f"  history: {config.storage.dtype} compression={config.storage.compression} S={config.sampling_interval}"
-        f" E-only={config.reconstruct_rhs} region={config.history_region}",
+        "=== DeepGPR compute preview ===",
+        f"device: {config.device}",
+        f"dx / dy / dz: {dx:g} / {dy:g} / {dz:g} m, dt: {config.dt:g} s",
+        f"extended grid (nx, ny, nz): ({config.nx}, {config.ny}, {config.nz}), CPML faces: {config.pml}",
+        f"time steps: {config.nt}, shots: {config.nstep}, sources / receivers per shot: {config.nsr} / {config.nrx}",
+        f"FDTD order / gradient mode: {config.fdtd_order} / {config.mode}",
+        f"source / receiver component: {config.source_direction} / {config.receiver_component}",
+        f"2D Ez-TM kernels: {'yes' if config.tm2d_fast_path else 'no'}",
+        f"history: save={config.save_wavefield_history}, storage={st.dtype}, compression={st.compression}"
+        + (f" (block {tuple(st.block_size)})" if st.compression == "int8" else "")
+        + f", sampling interval={config.sampling_interval}, region={config.history_region}",
+        f"R history: {'rebuilt from E (E-only history)' if est['reconstruct_rhs'] else 'stored'}",
+        f"gradient accumulator: {config.grad_accumulator}",
+        f"async offload: requested={config.use_async_offload}, effective=False (histories stay in device memory)",
+        f"forward wavefield save directory: {save_directory if save_directory is not None else 'disabled'}",
+        "memory estimate (tensor payload):",
     ]
-    lines += [f"  {k:>15s}: {v / 2**20:12.1f} MiB" for k, v in est.items()]
+    lines += [f"  {label}: {_mib(est[key])}" for label, key in _REPORT_ROWS]
+    lines += [
+        f"estimated peak HIP device memory: {_mib(est['estimated_device_peak'])}",
+        f"recommended device capacity with 20% margin: {_mib(est['recommended_device_capacity'])}",
+        "=== End DeepGPR compute preview ===",
+    ]
     return "\n".join(lines)

@@ -294,7 +294,9 @@ def prepare_compute(device, dx=None, dt=None, source_amplitudes=None, source_loc
         tm2d_fast_path=tm2d, grad_accumulator=gradient_accumulator)
     if print_parameters:
         from .memory import format_compute_preview
-        print(format_compute_preview(config))
+        print(format_compute_preview(config, source_waveforms=int(waveforms.shape[0]),
+                                     er_requires_grad=bool(eps_ext.requires_grad),
+                                     se_requires_grad=bool(sig_ext.requires_grad), save_directory=out_dir))
     tensors = SolverTensors(mu_r_pad=prepared.mr, eps_r_pad=prepared.ere, sigma_pad=prepared.see,
                             source_location=src_loc, receiver_location=rx_loc, pml_descriptors=tuple(descriptors),
                             pml_coefficients=tuple(coefficients))

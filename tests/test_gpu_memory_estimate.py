@@ -61,3 +61,27 @@ def test_estimate_bounds_the_allocator_peak(cuda_device, name):
     # within a few percent where the history dominates (measured on MI355X:
     # 0.2-0.7 %, the estimate above or below by the terms it leaves out)
     assert abs(held - priced) <= 0.03 * priced
+
+
+def test_compute_preview_reports_memory(cuda_device, capsys):
+    # restates tests/test_numerics.py:262-299 on the HIP device
+    import deepgpr_amd as dg
+    print("[gpu memory estimate] print_parameters", flush=True)
+    nx, ny, nt = 12, 16, 80
+    er = torch.full((nx, ny), 4.0, device=cuda_device, requires_grad=True)
+    se = torch.full((nx, ny), 2.0e-4, device=cuda_device, requires_grad=True)
+    source = dg.wavelet.ricker(3.0e8, nt, 3.0e-11, 3.0e-9).reshape(1, nt, 1).to(cuda_device)
+    capsys.readouterr()
+    dg.compute(device=cuda_device, dx=0.02, dt=3.0e-11, source_amplitudes=source,
+               source_location=torch.tensor([[[4, 6, 0]]], dtype=torch.int32, device=cuda_device),
+               receiver_location=torch.tensor([[[4, 10, 0]]], dtype=torch.int32, device=cuda_device),
+               er=er, se=se, pmlthick=3, model_gradient_sampling_interval=2, wavefield_storage_dtype=torch.float16,
+               fdtd_order=4, mode=2, print_parameters=True)
+    preview = capsys.readouterr().out
+    for line in ("=== DeepGPR compute preview ===", "dx / dy / dz", "FDTD order / gradient mode: 4 / 2",
+                 "saved E_saved and R_saved wavefields", "adjoint fields and CPML",
+                 "forward wavefield save directory: disabled", "estimated peak HIP device memory",
+                 "recommended device capacity with 20% margin", "=== End DeepGPR compute preview ==="):
+        assert line in preview, line
+    with pytest.raises(TypeError, match="print_parameters"):
+        dg.compute(device=cuda_device, print_parameters=1)

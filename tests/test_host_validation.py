@@ -112,3 +112,10 @@ def test_storage_type_bits():
     bits = st.encode_storage_type(cfg8, True, physical_history=True)
     assert bits & 0xF == 3 and (bits >> 8) & 63 == 4 and (bits >> 14) & 63 == 4 and (bits >> 20) & 63 == 4
     assert bits & (1 << 5)
+
+
+def test_print_parameters_must_be_a_bool():
+    # tests/test_numerics.py:301-303: checked before any model or device work
+    import deepgpr_amd as dg
+    with pytest.raises(TypeError, match="print_parameters"):
+        dg.compute(device="cpu", print_parameters=1)

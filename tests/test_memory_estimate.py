@@ -161,3 +161,36 @@ def test_cpml_term_matches_the_allocated_tensors():
     tensors = build_pml_phi(*faces, nstep, None, torch.device("cpu"))
     est = dg.estimate_compute_memory(**_common(nx=nx, ny=ny, nz=nz, nstep=nstep, pml=pml, mode=3))
     assert est["cpml_auxiliary_fields"] == 4 * sum(t.numel() for t in tensors)
+
+
+def _config(**over):
+    from deepgpr_amd.solver.autograd import SolverConfig
+    from deepgpr_amd.solver.storage import WavefieldStorageConfig
+    args = dict(device=CUDA, dtype=torch.float32, grid_spacing=(0.02, 0.02, 0.02), dt=3e-11, nx=18, ny=22, nz=1,
+                nt=80, nstep=1, nsr=1, nrx=1, pmlthick=torch.tensor([3, 3, 3, 3, 0, 0], dtype=torch.int32),
+                source_direction=2, receiver_component=2, sampling_interval=2,
+                storage=WavefieldStorageConfig(torch.float16, "none", None, "current", "current"),
+                save_wavefield_history=True, use_async_offload=False, fdtd_order=4, mode=2, debug=False,
+                tm2d_fast_path=True)
+    args.update(over)
+    return SolverConfig(**args)
+
+
+def test_preview_reports_the_estimate():
+    # the labels the reference's preview test looks for (tests/test_numerics.py:262-299),
+    # with the HIP device in place of the CPU
+    from deepgpr_amd.solver.memory import format_compute_preview
+    text = format_compute_preview(_config(), source_waveforms=1, er_requires_grad=True, se_requires_grad=True)
+    for line in ("=== DeepGPR compute preview ===", "dx / dy / dz", "FDTD order / gradient mode: 4 / 2",
+                 "saved E_saved and R_saved wavefields", "adjoint fields and CPML",
+                 "forward wavefield save directory: disabled", "estimated peak HIP device memory",
+                 "recommended device capacity with 20% margin", "=== End DeepGPR compute preview ==="):
+        assert line in text, line
+    est = dg.estimate_compute_memory(device=CUDA, nx=18, ny=22, nz=1, nt=80, nstep=1, nsr=1, nrx=1,
+                                     source_waveforms=1, pml=[3, 3, 3, 3, 0, 0], mode=2, sampling_interval=2,
+                                     storage_dtype=torch.float16, use_async_offload=False, er_requires_grad=True,
+                                     se_requires_grad=True, reconstruct_rhs=False)
+    assert f"estimated peak HIP device memory: {est['estimated_device_peak'] / 2**20:.2f} MiB" in text
+    saved = format_compute_preview(_config(), source_waveforms=1, er_requires_grad=False, se_requires_grad=False,
+                                   save_directory="/tmp/run")
+    assert "forward wavefield save directory: /tmp/run" in saved
