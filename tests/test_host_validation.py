@@ -119,3 +119,46 @@ def test_print_parameters_must_be_a_bool():
     import deepgpr_amd as dg
     with pytest.raises(TypeError, match="print_parameters"):
         dg.compute(device="cpu", print_parameters=1)
+
+
+def test_cpml_state_shapes_are_checked_before_the_solve():
+    # tests/test_numerics.py:371-404, through build_pml_phi (the check compute()
+    # makes before its native call; this build has no CPU solve)
+    import torch
+    from deepgpr_amd.solver.pml import build_pml_phi, buildpmlcoeffs
+    nx, ny, nz, nstep = 12, 14, 1, 1
+    er = torch.full((nx, ny, nz), 4.0)
+    mr = torch.ones((nx + 1, ny + 1, nz + 1))
+    faces = buildpmlcoeffs(er, mr, 3.0e-11, 0.02, nx, ny, nz, torch.tensor([2, 2, 2, 2, 0, 0], dtype=torch.int32),
+                           torch.device("cpu"), torch.float32)[:6]
+    state = list(build_pml_phi(*faces, nstep, None, torch.device("cpu")))
+    assert len(build_pml_phi(*faces, nstep, tuple(state), torch.device("cpu"))) == 24
+    state[0] = state[0][..., :-1]
+    with pytest.raises(ValueError, match="CPML face 0 component 0"):
+        build_pml_phi(*faces, nstep, tuple(state), torch.device("cpu"))
+
+
+def test_anisotropic_cpml_coefficients_use_axis_spacing():
+    # tests/test_numerics.py:406-429
+    import torch
+    from deepgpr_amd.solver.pml import buildpmlcoeffs
+    nx, ny, nz = 10, 12, 14
+    c = buildpmlcoeffs(torch.full((nx, ny, nz), 4.0), torch.ones((nx + 1, ny + 1, nz + 1)), 2.0e-11,
+                       [0.02, 0.015, 0.01], nx, ny, nz, torch.tensor([2] * 6, dtype=torch.int32),
+                       torch.device("cpu"), torch.float32)
+    assert not torch.equal(c[6], c[10]) and not torch.equal(c[10], c[14])
+
+
+def test_initialization_extends_pml_outside_the_acquisition():
+    # tests/test_numerics.py:701-716
+    import warnings
+    import torch
+    from deepgpr_amd.preprocessing.model_setup import initialization
+    er = torch.full((12, 14), 4.0)
+    with warnings.catch_warnings(record=True) as captured:
+        warnings.simplefilter("always")
+        result = initialization(torch.device("cpu"), er, torch.zeros_like(er), None, torch.zeros((1, 20, 1)),
+                                torch.tensor([[[2, 7, 0]]], dtype=torch.int32),
+                                torch.tensor([[[6, 7, 0]]], dtype=torch.int32), 0.02, 3.0e-11, 2, 2)
+    assert not captured
+    assert tuple(result[2:5]) == (16, 18, 1)
