@@ -106,8 +106,12 @@ def _sampling(requested, automatic: bool, waveforms, dt, report: bool) -> int:
         return bound
     if requested > bound:
         warnings.warn(f"model_gradient_sampling_interval={requested} exceeds the recommended bound {bound} = "
-                      f"floor(1 / (4 f_max dt)) for the source spectrum (f_max={f_max:.4e} Hz, dt={float(dt):.4e} s).",
-                      RuntimeWarning, stacklevel=3)
+                      f"floor(1 / (4 f_max dt)) for the source spectrum (f_max={f_max:.4e} Hz, dt={float(dt):.4e} s). "
+                      "Above the bound the sampled gradient loses accuracy quickly; pass 'auto' or an interval "
+                      "within it.", RuntimeWarning, stacklevel=3)
+    else:
+        _LOG.info("model_gradient_sampling_interval=%d samples the gradient in time, within the recommended bound "
+                  "%d (f_max=%.4e Hz).", requested, bound, f_max)
     return int(requested)
 
 
