@@ -91,7 +91,8 @@ def _int8_block(compression: str, block, spatial_mode: int, reduction: str):
         reduction = ("cub_block" if compression == "int8"
                      and math.prod(resolved) == INT8_SPECIALISED_REDUCTION_VOLUME else "current")  # type: ignore
     if compression == "int8" and reduction != "current" and math.prod(resolved) != INT8_SPECIALISED_REDUCTION_VOLUME:  # type: ignore
-        raise ValueError("The cub_block and warp_shuffle INT8 reduction backends require a 64-voxel block.")
+        raise ValueError("The cub_block and warp_shuffle INT8 reduction backends need a compression block of "
+                         "exactly 64 voxels.")
     return resolved, reduction
 
 
