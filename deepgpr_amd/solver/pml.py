@@ -166,7 +166,15 @@ def build_pml_coeffs(eps_r, mu_r, dt, dx, nx, ny, nz, pmlthick, device, dtype, *
     return (*descriptors, *coefficients)
 
 
-buildpmlcoeffs = build_pml_coeffs
+def buildpmlcoeffs(*args, **kwargs):
+    """The older name of build_pml_coeffs, which also takes the materials as
+    ``er`` / ``mr`` keywords (src/DeepGPR/solver/pml.py:301-307)."""
+    for old, new in (("er", "eps_r"), ("mr", "mu_r")):
+        if old in kwargs:
+            if new in kwargs:
+                raise TypeError(f"Specify only one of {new} and its older name {old}.")
+            kwargs[new] = kwargs.pop(old)
+    return build_pml_coeffs(*args, **kwargs)
 
 
 def expected_phi_shapes(face: int, descriptor: torch.Tensor, nstep: int) -> Tuple[Optional[ShapeTuple], ...]:
