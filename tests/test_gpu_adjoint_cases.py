@@ -1,9 +1,11 @@
-"""Adjoint cases of the reference's tests/test_discrete_adjoint.py:204-284 and
-585-800 through compute() on the HIP device: coincident sources and
-receivers, Taylor checks of full-model material directions without and with
-CPML, the choice of the 2D Ez-TM kernels and their identity with the general
-kernels, and finite results for a large all-face CPML with FP16 histories and
-for an order-8 asymmetric CPML with an incomplete sampling block."""
+"""Adjoint cases of the reference's tests/test_discrete_adjoint.py:204-284,
+317-335, 475-547 and 585-800 through compute() on the HIP device: coincident
+sources and receivers, Taylor checks of full-model material directions
+without and with CPML, the choice of the 2D Ez-TM kernels and their identity
+with the general kernels, finite results for a large all-face CPML with FP16
+histories and for an order-8 asymmetric CPML with an incomplete sampling
+block, the older keyword names, the source-waveform gradient and the weight
+of an incomplete sampling block."""
 import contextlib
 
 import pytest
@@ -189,3 +191,68 @@ def test_order_8_asymmetric_cpml_and_incomplete_sampling_block(cuda_device):
     torch.cuda.synchronize()
     assert bool(torch.isfinite(receiver).all())
     assert bool(torch.isfinite(eps_r.grad).all()) and bool(torch.isfinite(sigma.grad).all())
+
+
+def test_older_and_current_keyword_names_give_the_same_bits(cuda_device):
+    # tests/test_discrete_adjoint.py:317-335
+    import deepgpr_amd as dg
+    eps_r = torch.full((10, 12), 4.0, device=cuda_device)
+    sigma = torch.zeros_like(eps_r)
+    location = torch.tensor([[[5, 6, 0]]], dtype=torch.int32, device=cuda_device)
+    common = dict(device=cuda_device, dx=0.02, dt=3.0e-11,
+                  source_amplitudes=torch.linspace(-0.5, 0.5, 20, device=cuda_device).reshape(1, 20, 1),
+                  source_location=location, receiver_location=location, pmlthick=0, fdtd_order=2, mode=2)
+    old = dg.compute(er=eps_r, se=sigma, reciever_direction=2, **common)[-1]
+    new = dg.compute(eps_r=eps_r, sigma=sigma, receiver_component=2, **common)[-1]
+    assert torch.equal(old, new)
+
+
+def test_source_waveform_gradient_matches_finite_differences(cuda_device):
+    # tests/test_discrete_adjoint.py:475-508
+    import math
+    import deepgpr_amd as dg
+    print("[gpu adjoint cases] source waveform gradient", flush=True)
+    nx, ny, nt = 12, 14, 60
+    arguments = dict(device=cuda_device, dx=0.02, dt=3.0e-11,
+                     source_location=torch.tensor([[[5, 5, 0]]], dtype=torch.int32, device=cuda_device),
+                     receiver_location=torch.tensor([[[5, 8, 0]]], dtype=torch.int32, device=cuda_device),
+                     er=torch.full((nx, ny), 4.0, device=cuda_device),
+                     se=torch.zeros((nx, ny), device=cuda_device), pmlthick=0, fdtd_order=2, mode=2)
+    base = dg.wavelet.ricker(4.0e8, nt, 3.0e-11, 2.0e-9).reshape(1, nt, 1).to(cuda_device)
+
+    def objective(source):
+        return 0.5 * dg.compute(source_amplitudes=source, **arguments)[-1].square().sum()
+
+    source = base.clone().requires_grad_(True)
+    objective(source).backward()
+    direction = torch.randn(source.shape, generator=torch.Generator().manual_seed(41)).to(cuda_device)
+    direction /= direction.norm()
+    adjoint = float((source.grad * direction).sum())
+    best = math.inf
+    for step in (2.0e-3, 1.0e-3, 5.0e-4):
+        with torch.no_grad():
+            fd = float((objective(base + step * direction) - objective(base - step * direction)) / (2.0 * step))
+        best = min(best, abs(adjoint - fd) / max(abs(adjoint), abs(fd), 1.0e-20))
+    assert best < 8.0e-3
+
+
+def test_incomplete_sampling_block_uses_its_actual_length(cuda_device):
+    # tests/test_discrete_adjoint.py:523-547: an interval of nt and of nt + 5
+    # both save only step 0, whose weight is the block's actual length
+    import deepgpr_amd as dg
+    nx, ny, nt = 12, 14, 11
+    eps_base = torch.full((nx, ny), 4.0, device=cuda_device)
+    sigma = torch.full_like(eps_base, 2.0e-4)
+    source = torch.linspace(-0.5, 0.8, nt, device=cuda_device).reshape(1, nt, 1)
+
+    def gradient(interval):
+        eps_r = eps_base.clone().requires_grad_(True)
+        receiver = dg.compute(device=cuda_device, dx=0.02, dt=3.0e-11, source_amplitudes=source,
+                              source_location=torch.tensor([[[5, 5, 0]]], dtype=torch.int32, device=cuda_device),
+                              receiver_location=torch.tensor([[[5, 7, 0]]], dtype=torch.int32, device=cuda_device),
+                              er=eps_r, se=sigma, pmlthick=0, fdtd_order=2, mode=2,
+                              model_gradient_sampling_interval=interval, wavefield_storage_dtype=torch.float32)[-1]
+        receiver.square().sum().backward()
+        return eps_r.grad
+
+    assert torch.equal(gradient(nt), gradient(nt + 5))
