@@ -1,6 +1,8 @@
 """History options through compute() on the HIP device (restates the
-reference's tests/test_wavefield_compression.py:32-137 and 205-255): no
-history, the INT8 reduction backends, partial INT8 tiles in 3D."""
+reference's tests/test_wavefield_compression.py:32-137 and 205-255 and
+tests/test_cuda_conversion_backends.py:131-210): no history, the INT8
+reduction backends, partial INT8 tiles in 3D, and the FP16/BF16 conversion
+backends."""
 import pytest
 import torch
 
@@ -96,3 +98,44 @@ def test_3d_partial_int8_tiles(cuda_device):
     torch.cuda.synchronize(cuda_device)
     assert result[0].dtype == torch.int8
     assert bool(torch.isfinite(eps_r.grad).all()) and bool(eps_r.grad.abs().sum() > 0)
+
+
+def _run_conversion(dev, storage_dtype, backend):
+    # tests/test_cuda_conversion_backends.py:131-174
+    import deepgpr_amd as dg
+    nx, ny, nt = 24, 28, 120
+    x = torch.linspace(0.0, 1.0, nx, device=dev)[:, None]
+    y = torch.linspace(0.0, 1.0, ny, device=dev)[None, :]
+    eps_r = (4.0 + 0.35 * x + 0.15 * y).requires_grad_(True)
+    sigma = (2.0e-4 + 0.3e-4 * x + 0.2e-4 * y).requires_grad_(True)
+    source = dg.wavelet.ricker(3.5e8, nt, 2.5e-11, 2.0e-9, device=dev).reshape(1, nt, 1)
+    result = dg.compute(device=dev, dx=(0.020, 0.016, 0.012), dt=2.5e-11, source_amplitudes=source,
+                        source_location=torch.tensor([[[10, 8, 0]]], dtype=torch.int32, device=dev),
+                        receiver_location=torch.tensor([[[8, 14, 0], [12, 18, 0]]], dtype=torch.int32, device=dev),
+                        eps_r=eps_r, sigma=sigma, pmlthick=4, fdtd_order=4, mode=2, model_gradient_sampling_interval=1,
+                        wavefield_storage_dtype=storage_dtype, wavefield_conversion_backend=backend)
+    data = result[-1]
+    (0.5 * (data / data.detach().abs().max().clamp_min(1.0e-12)).square().sum()).backward()
+    return {"history": result[0].detach(), "data": data.detach(), "eps": eps_r.grad, "sigma": sigma.grad}
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
+def test_conversion_backends_give_the_same_solve(cuda_device, dtype):
+    # the reference allows 5e-7 between its backends; these share one
+    # round-to-nearest-even conversion, so every quantity is equal
+    print(f"[gpu history cases] conversion backends {dtype}", flush=True)
+    legacy = _run_conversion(cuda_device, dtype, "legacy")
+    for backend in ("native_scalar", "native_vec2"):
+        candidate = _run_conversion(cuda_device, dtype, backend)
+        for name in legacy:
+            assert torch.equal(candidate[name], legacy[name]), (backend, name)
+
+
+def test_conversion_backend_is_refused_where_it_does_not_apply(cuda_device):
+    import deepgpr_amd as dg
+    with pytest.raises(ValueError, match="only selectable"):
+        dg.compute(device=cuda_device, dx=0.02, dt=1.0e-11, source_amplitudes=torch.zeros((1, 2, 1), device=cuda_device),
+                   source_location=torch.tensor([[[2, 2, 0]]], dtype=torch.int32, device=cuda_device),
+                   receiver_location=torch.tensor([[[2, 2, 0]]], dtype=torch.int32, device=cuda_device),
+                   eps_r=torch.full((6, 6), 4.0, device=cuda_device), sigma=torch.full((6, 6), 2.0e-4, device=cuda_device),
+                   pmlthick=1, wavefield_storage_dtype=torch.float32, wavefield_conversion_backend="native_scalar")
