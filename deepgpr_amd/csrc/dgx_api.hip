@@ -1558,6 +1558,10 @@ int run_backward(const dgx_solve* P, hipStream_t stream)
     mg.want_sig = P->sigma_requires_grad == 1;
     mg.dt = P->dt;
     mg.box = pl.box;
+    if (pl.box.nz > 0 && pl.box.ny > 0) {
+        mg.div_z = make_div((unsigned)pl.box.nz);
+        mg.div_y = make_div((unsigned)pl.box.ny);
+    }
     const float geo = want_src ? g.d[P->source_component] / (g.d[0] * g.d[1] * g.d[2]) : 0.0f;
     Tm2dState tms;
     if (tm && tms.setup(pl, P, stream, fused, tg.bi)) {

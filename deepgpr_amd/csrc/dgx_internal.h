@@ -67,6 +67,7 @@ struct MatGrad {
     int want_eps, want_sig;
     float dt;
     Box box;
+    Div div_z, div_y;  // division by box.nz and box.ny, for boxes of fewer than 2^31 cells
 };
 
 struct StepArgs {

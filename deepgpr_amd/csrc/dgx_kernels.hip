@@ -551,9 +551,20 @@ __global__ __launch_bounds__(256) void k_material_gradient(Grid g, CF3 L, MatGra
     long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const Box b = mg.box;
     if (h >= b.cells) return;
-    const int ix = (int)(h / ((long long)b.ny * b.nz)) + b.x0;
-    const int iy = (int)((h / b.nz) % b.ny) + b.y0;
-    const int iz = (int)(h % b.nz) + b.z0;
+    int ix, iy, iz;
+    if (b.cells <= 0x7fffffffLL) {
+        // 32-bit multiply-and-shift divisions instead of three 64-bit ones
+        const unsigned q = (unsigned)h;
+        const unsigned row = div_q(q, mg.div_z);
+        const unsigned px = div_q(row, mg.div_y);
+        iz = (int)(q - row * (unsigned)b.nz) + b.z0;
+        iy = (int)(row - px * (unsigned)b.ny) + b.y0;
+        ix = (int)px + b.x0;
+    } else {
+        ix = (int)(h / ((long long)b.ny * b.nz)) + b.x0;
+        iy = (int)((h / b.nz) % b.ny) + b.y0;
+        iz = (int)(h % b.nz) + b.z0;
+    }
     const int c[3] = {ix, iy, iz};
     const long long m = (long long)ix * g.st0 + (long long)iy * g.N[2] + iz;
     const long long out = ((long long)ix * (g.N[1] - 1) + iy) * (g.N[2] - 1) + iz;
