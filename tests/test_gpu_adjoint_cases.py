@@ -4,8 +4,9 @@ sources and receivers, Taylor checks of full-model material directions
 without and with CPML, the choice of the 2D Ez-TM kernels and their identity
 with the general kernels, finite results for a large all-face CPML with FP16
 histories and for an order-8 asymmetric CPML with an incomplete sampling
-block, the older keyword names, the source-waveform gradient and the weight
-of an incomplete sampling block."""
+block, the older keyword names, the source-waveform gradient, the weight
+of an incomplete sampling block, and a long 3D CPML backward
+(tests/test_numerics.py:717-754)."""
 import contextlib
 
 import pytest
@@ -256,3 +257,25 @@ def test_incomplete_sampling_block_uses_its_actual_length(cuda_device):
         return eps_r.grad
 
     assert torch.equal(gradient(nt), gradient(nt + 5))
+
+
+def test_long_3d_cpml_backward_remains_finite(cuda_device):
+    # tests/test_numerics.py:717-754: 1500 steps, sampling interval 10
+    import deepgpr_amd as dg
+    print("[gpu adjoint cases] long 3D CPML backward", flush=True)
+    nx, ny, nz, nt = 20, 24, 24, 1500
+    er = torch.full((nx, ny, nz), 4.0, dtype=torch.float32, device=cuda_device, requires_grad=True)
+    se = torch.zeros_like(er, requires_grad=True)
+    source = dg.wavelet.ricker(5.0e8, nt, 2.0e-11, 2.0e-9).reshape(1, nt, 1).to(cuda_device)
+    result = dg.compute(device=cuda_device, dx=0.02, dt=2.0e-11, source_amplitudes=source,
+                        source_location=torch.tensor([[[10, 7, 12]]], dtype=torch.int32, device=cuda_device),
+                        receiver_location=torch.tensor([[[8, 7, 10], [12, 7, 14]]], dtype=torch.int32,
+                                                       device=cuda_device),
+                        er=er, se=se, pmlthick=5, model_gradient_sampling_interval=10,
+                        wavefield_storage_dtype=torch.float32, fdtd_order=2, mode=3, debug=True)
+    receiver = result[-1]
+    receiver.square().mean().backward()
+    assert bool(torch.isfinite(receiver).all())
+    assert er.grad is not None and bool(torch.isfinite(er.grad).all()) and er.grad.shape == er.shape
+    assert se.grad is not None and bool(torch.isfinite(se.grad).all()) and se.grad.shape == se.shape
+    assert int(torch.count_nonzero(er.grad[:6])) > 0
