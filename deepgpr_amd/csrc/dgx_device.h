@@ -627,14 +627,26 @@ struct Int8View {
     long long tiles;          // tiles per shot
 };
 
+// Tile of box cell (x, y, z) (box-relative coordinates).
+__device__ __forceinline__ long long int8_tile(const Int8View& v, int x, int y, int z)
+{
+    return ((long long)(x / v.bx) * v.nby + y / v.by) * v.nbz + z / v.bz;
+}
+
+// Value of box cell h, whose tile the caller has formed once (int8_tile).
+__device__ __forceinline__ float int8_load_at(const Int8View& v, const Box& b, int comp, int shot, long long h,
+                                              long long tile)
+{
+    float sc = v.scale[comp * v.s_cstride + (long long)shot * v.tiles + tile];
+    return sc * (float)v.q[comp * v.q_cstride + (long long)shot * b.cells + h];
+}
+
 __device__ __forceinline__ float int8_load(const Int8View& v, const Box& b, int comp, int shot, long long h)
 {
     int x = (int)(h / ((long long)b.ny * b.nz));
     int y = (int)((h / b.nz) % b.ny);
     int z = (int)(h % b.nz);
-    long long tile = ((long long)(x / v.bx) * v.nby + y / v.by) * v.nbz + z / v.bz;
-    float sc = v.scale[comp * v.s_cstride + (long long)shot * v.tiles + tile];
-    return sc * (float)v.q[comp * v.q_cstride + (long long)shot * b.cells + h];
+    return int8_load_at(v, b, comp, shot, h, int8_tile(v, x, y, z));
 }
 
 }  // namespace dgx

@@ -572,6 +572,12 @@ __global__ __launch_bounds__(256) void k_material_gradient(Grid g, CF3 L, MatGra
     const bool active = mg.sig[m] <= kPecSigma;
     const int comps = mg.mode == 3 ? 3 : 1;
     const float wgt = (float)mg.weight;
+    // the INT8 tiles of this cell, formed once for every shot and component
+    long long tile_e = 0, tile_s = 0;
+    if (mg.kind == kInt8) {
+        tile_e = int8_tile(mg.e8, ix - b.x0, iy - b.y0, iz - b.z0);
+        tile_s = int8_tile(mg.s8, ix - b.x0, iy - b.y0, iz - b.z0);
+    }
     ACC ge = 0, gs = 0;
     for (int s = 0; s < g.nshot; ++s) {
         const long long f = (long long)s * g.cells + m;
@@ -581,8 +587,8 @@ __global__ __launch_bounds__(256) void k_material_gradient(Grid g, CF3 L, MatGra
             const bool up = e_updated(g, comp, c);
             float e_old, sec;
             if (mg.kind == kInt8) {
-                e_old = int8_load(mg.e8, b, ci, s, h);
-                sec = int8_load(mg.s8, b, ci, s, h);
+                e_old = int8_load_at(mg.e8, b, ci, s, h, tile_e);
+                sec = int8_load_at(mg.s8, b, ci, s, h, tile_s);
             } else {
                 e_old = load_hist(mg.eframe, ci * mg.e_cstride + base, mg.kind);
                 sec = load_hist(mg.sframe, ci * mg.s_cstride + base, mg.kind);
