@@ -412,6 +412,10 @@ Int8Write int8_writer(const Plan& pl, void* packed, int frames, int frame)
     w.tiles = q.tiles;
     w.mode = pl.comps == 3 ? 3 : 2;
     w.box = pl.box;
+    w.div_nbz = make_div((unsigned)q.nbz);
+    w.div_tplane = make_div((unsigned)(q.nby * q.nbz));
+    w.div_bz = make_div((unsigned)w.bz);
+    w.div_bplane = make_div((unsigned)(w.by * w.bz));
     return w;
 }
 

@@ -41,6 +41,8 @@ struct Int8Write {
     long long tiles;
     int mode;
     Box box;
+    // divisions by nbz, nby * nbz, bz and by * bz (tile counts below 2^31)
+    Div div_nbz, div_tplane, div_bz, div_bplane;
 };
 
 struct ReceiverGroups {

@@ -475,8 +475,11 @@ __global__ __launch_bounds__(256) void k_quant_int8_w64(Grid g, CF3 E, const flo
     const int tile = blockIdx.x * 4 + (int)(threadIdx.x >> 6), s = blockIdx.y, ci = blockIdx.z;
     if (tile >= w.tiles) return;  // whole wavefronts
     const int comp = w.mode == 3 ? ci : 2;
-    const int tz = tile % w.nbz, ty = (tile / w.nbz) % w.nby, tx = tile / (w.nbz * w.nby);
-    const int lx = local / (w.by * w.bz), ly = (local / w.bz) % w.by, lz = local % w.bz;
+    // tile and in-tile coordinates by multiply-and-shift divisions
+    const unsigned trow = div_q((unsigned)tile, w.div_nbz), tx = div_q((unsigned)tile, w.div_tplane);
+    const int tz = tile - (int)(trow * (unsigned)w.nbz), ty = (int)(trow - tx * (unsigned)w.nby);
+    const unsigned lrow = div_q((unsigned)local, w.div_bz), lx = div_q((unsigned)local, w.div_bplane);
+    const int lz = local - (int)(lrow * (unsigned)w.bz), ly = (int)(lrow - lx * (unsigned)w.by);
     const int x = tx * w.bx + lx, y = ty * w.by + ly, z = tz * w.bz + lz;
     const bool valid = x < w.box.nx && y < w.box.ny && z < w.box.nz;
     const long long h = valid ? ((long long)x * w.box.ny + y) * w.box.nz + z : 0;
